@@ -1,0 +1,173 @@
+/*
+ * linqt_amd.h -- C ABI of the MI355X (gfx950) Chebyshev-moment engine.
+ *
+ * The reference exposes this path as C++ classes backed by MKL/FFTW; each
+ * entry point below replaces one of them (cited per function).  Plain
+ * pointers and sizes only.  Complex numbers are interleaved (re, im)
+ * doubles.  Every function returns LQ_OK (0) or a negative LQ_ERR_* code and
+ * never throws; lq_last_error() gives the calling thread's last message.
+ *
+ * Threading: one lq_ctx owns one HIP stream, its device buffers and scratch.
+ * Calls on one context must not overlap; use one context per host thread.
+ *
+ * Layouts on the device:
+ *   - a block of k vectors of length n is stored interleaved, element (i, v)
+ *     at index i*k + v ("[n][k]");
+ *   - state arrays (phi) are state-major, state r at phi[2*(r*n + i)],
+ *     as the reference's `vector` state file holds them
+ *     (include/quantum_states.hpp:155-183); the solvers accept phi in host
+ *     memory or in device memory of the context's GPU (lq_malloc);
+ *   - 2D moment tables are row major, mu(m0, m1) at m0*M1 + m1
+ *     (include/chebyshev_moments.hpp:255).
+ */
+#ifndef LINQT_AMD_H
+#define LINQT_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LQ_ABI_VERSION 1
+
+enum lq_status {
+  LQ_OK = 0,
+  LQ_ERR_ARG = -1,         /* invalid argument / size / handle */
+  LQ_ERR_HIP = -2,         /* HIP runtime error */
+  LQ_ERR_NOMEM = -3,       /* device allocation failed */
+  LQ_ERR_UNSUPPORTED = -4, /* a case whose reference result is undefined */
+  LQ_ERR_NODEV = -5        /* no GPU / HIP runtime unavailable */
+};
+
+/* SparseMatrixType descriptor fill (include/sparse_matrix.hpp:50-55). */
+enum lq_fill {
+  LQ_FILL_HERMITIAN_UPPER = 0, /* MKL HERMITIAN + FILL_MODE_UPPER: the reference's only mode */
+  LQ_FILL_GENERAL = 1          /* entries used as stored */
+};
+
+/* chebyshev::formula (include/Kubo_solver_FFT.hpp:17-21). */
+enum lq_formula { LQ_KUBO_GREENWOOD = 0, LQ_KUBO_BASTIN = 1 };
+
+/* 1D moment algorithm. */
+enum lq_algo1d {
+  LQ_ALGO_DIRECT = 0,  /* mu_m = <phi|T_m>, the reference's loop */
+  LQ_ALGO_DOUBLING = 1 /* mu_2n = 2<T_n|T_n> - mu_0, mu_2n+1 = 2<T_n+1|T_n> - mu_1 (OP = identity only) */
+};
+
+/* Contraction precision of the 2D table. */
+enum lq_precision {
+  LQ_PREC_FP64 = 0 /* FP64 MFMA */
+};
+
+typedef struct lq_ctx_s *lq_ctx;
+typedef struct lq_csr_s *lq_csr;
+
+/* Per-context kernel timing (HIP events on the context stream), summed since
+ * the last lq_ctx_reset_timing.  Bytes / flops are algorithmic counts. */
+typedef struct lq_timing {
+  double spmv_ms;        /* recursion SpMV/SpMM and operator applies */
+  double contract_ms;    /* moment contraction (Lᴴ R) incl. split-K reduce */
+  double post_ms;        /* epilogues, Jackson, energy-grid sums */
+  double spmv_bytes;     /* sum of 20*nnz + 4*(n+1) + (48 or 32)*k*n per launch */
+  double contract_flops; /* sum of 8*M0*M1*K per contraction */
+  long long spmv_launches;
+  long long contract_launches;
+} lq_timing;
+
+/* ---- runtime ----------------------------------------------------------- */
+const char *lq_last_error(void);
+int lq_abi_version(void);
+int lq_device_count(int *count);
+int lq_ctx_create(int device, lq_ctx *out);
+int lq_ctx_destroy(lq_ctx ctx);
+int lq_ctx_synchronize(lq_ctx ctx);
+int lq_ctx_set_timing(lq_ctx ctx, int enabled);
+int lq_ctx_reset_timing(lq_ctx ctx);
+int lq_ctx_get_timing(lq_ctx ctx, lq_timing *out);
+/* Device memory of the context's device (plain device pointers). */
+int lq_malloc(lq_ctx ctx, size_t bytes, void **dptr);
+int lq_free(lq_ctx ctx, void *dptr);
+int lq_memcpy_h2d(lq_ctx ctx, void *dst, const void *src, size_t bytes);
+int lq_memcpy_d2h(lq_ctx ctx, void *dst, const void *src, size_t bytes);
+
+/* ---- sparse operators ---------------------------------------------------- */
+/* Replaces SparseMatrixType::ConvertFromCSR (src/mkl_sparse_matrix.cpp:39-54)
+ * + SparseMatrixBuilder::BuildOPFromCSRFile (include/sparse_matrix.hpp:100-109).
+ * Host CSR in (zero based); the operator is Hermitian-completed (for
+ * LQ_FILL_HERMITIAN_UPPER) and uploaded in a sliced-ELL layout. */
+int lq_csr_create(lq_ctx ctx, int n, int nnz, const int *rowptr, const int *col,
+                  const double *val, int fill, lq_csr *out);
+int lq_csr_destroy(lq_csr A);
+/* Replaces SparseMatrixType::Rescale (src/mkl_sparse_matrix.cpp:73-97):
+ * A <- a*A + b*I, as Moments::Rescale2ChebyshevDomain calls it with
+ * a = 1/HalfWidth, b = -BandCenter/HalfWidth (include/chebyshev_moments.hpp:95-99). */
+int lq_csr_rescale(lq_csr A, double a_re, double a_im, double b_re, double b_im);
+/* n, completed nnz, stored (padded) entries. */
+int lq_csr_info(lq_csr A, int *n, long long *nnz, long long *nnz_stored);
+
+/* Replaces SparseMatrixType::Multiply(a, x, b, y) (src/mkl_sparse_matrix.cpp:56-69)
+ * and BatchMultiply (:102-106): y <- a*A*x + b*y for k interleaved vectors.
+ * x, y are DEVICE pointers ([n][k]); b == 0 overwrites y. */
+int lq_spmv(lq_csr A, double a_re, double a_im, const double *x, double b_re, double b_im,
+            double *y, int k);
+
+/* Replaces chebyshev::ComputeMomTable (src/chebyshev_solver.cpp:26-42):
+ * sub[m0*ldsub + m1] = sum_k conj(L[m0*ldl + k]) * R[m1*ldr + k] for
+ * m0 < ML, m1 < MR, k < K.  All DEVICE pointers; FP64 MFMA. */
+int lq_compute_mom_table(lq_ctx ctx, const double *L, long long ldl, int ML, const double *R,
+                         long long ldr, int MR, long long K, double *sub, int ldsub);
+
+/* ---- moment solvers (host phi in, host moments out) ---------------------- */
+/* Replaces chebyshev::SpectralMoments (src/chebyshev_solver.cpp:176-200).
+ * H already rescaled.  OP == NULL is the identity (SparseMatrixType "1").
+ * mu (M complex) is accumulated into, as the reference's Moments1D is. */
+int lq_spectral_moments(lq_csr H, lq_csr OP, const double *phi, int R, int M, int algo,
+                        double *mu);
+
+/* Replaces chebyshev::CorrelationExpansionMoments (src/chebyshev_solver.cpp:45-172)
+ * for nopr right operators sharing one left operator: table o is
+ * mu_o(m0,m1) = <phi| OPL T_m0 OPR_o T_m1 |phi>, scaled by 4/R with 1/2 on
+ * m0 = 0 and m1 = 0 and made Hermitian as at chebyshev_solver.cpp:149-162.
+ * batch <= 0 is the default BATCH_SIZE = M0; other values reproduce the
+ * reference's blocked recursion (kpm_noneqop.hpp:27-39, chebyshev_vectors.cpp:4-21).
+ * mu: nopr tables of M0*M1 complex, in/out with the reference's semantics
+ * (accumulated into unless padding to a multiple of batch happens). */
+int lq_correlation_moments(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, const double *phi,
+                           int R, int M0, int M1, int batch, int precision, double *mu);
+
+/* Replaces Kubo_solver_FFT::compute (src/Kubo_solver_FFT.cpp:12-136 with
+ * Greenwood_FFTs.cpp / Bastin_FFTs.cpp): final_data (nopr x 2*nump complex)
+ * receives the mean over the R states of r_data.  Requires M <= nump (the
+ * reference overflows its DFT inputs otherwise). */
+int lq_kubo_fft(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, const double *phi, int R,
+                int M, int num_sections, int nump, int formula, int precision, double *final_data);
+
+/* Replaces Kubo_solver_FFT_postProcess::{Greenwood,Bastin}_postProcess
+ * (src/Kubo_solver_FFT_postProcess.cpp:120-397): energies (a*E - Ec,
+ * ascending) and sigma as the output file's two columns; integrand (Bastin,
+ * may be NULL) as the `_integrand` file's second column. */
+int lq_kubo_post(lq_ctx ctx, int formula, const double *final_data, int nump, int n, double W,
+                 double Ec, double *E, double *sigma, double *integrand);
+
+/* ---- Jackson kernels and energy-grid post-processors --------------------- */
+/* Moments1D::ApplyJacksonKernel (src/chebyshev_moments1D.cpp:68-87);
+ * *M_out receives the truncated count, mu compacted in place. */
+int lq_jackson_1d(lq_ctx ctx, double *mu, int M, double W, double broad_meV, int *M_out);
+/* Moments2D::ApplyJacksonKernel (src/chebyshev_moments2D.cpp:87-117). */
+int lq_jackson_2d(lq_ctx ctx, double *mu, int M0, int M1, double W, double b0_meV,
+                  double b1_meV, int *M0_out, int *M1_out);
+/* spectralFunctionFromChebmom_FFTgrid.cpp:35-73: num_div rows. */
+int lq_dos_fftgrid(lq_ctx ctx, const double *mu, int M, int n, double W, double Ec, int num_div,
+                   double *E, double *rho);
+/* kuboGreenwoodFromChebmom_FFTgrid.cpp:35-76: num_div-1 rows. */
+int lq_greenwood_fftgrid(lq_ctx ctx, const double *mu, int M0, int M1, int n, double W, double Ec,
+                         int num_div, double *E, double *sigma);
+/* kuboBastinFromChebmom_FFTgrid.cpp:57-106: num_div-1 rows (cumulative). */
+int lq_bastin_fftgrid(lq_ctx ctx, const double *mu, int M0, int M1, int n, double W, double Ec,
+                      int num_div, double *E, double *sigma);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

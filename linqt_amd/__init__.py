@@ -1,0 +1,319 @@
+"""linqt_amd -- MI355X (gfx950) engine for LinQT's Chebyshev-moment path.
+
+Python mirror of the reference's solver interface over the C ABI
+(``include/linqt_amd.h``).  Names and argument meanings follow the
+reference's C++ API:
+
+* :class:`SparseMatrix` -- ``SparseMatrixType`` (Hermitian/upper CSR,
+  ``Multiply``, ``Rescale``), ``include/sparse_matrix.hpp:47-87``;
+* :func:`spectral_moments` -- ``chebyshev::SpectralMoments``
+  (``src/chebyshev_solver.cpp:176-200``);
+* :func:`correlation_expansion_moments` --
+  ``chebyshev::CorrelationExpansionMoments`` (``src/chebyshev_solver.cpp:86-172``);
+* :func:`kubo_fft` / :func:`kubo_post` -- ``Kubo_solver_FFT::compute`` and
+  its post-processing (``src/Kubo_solver_FFT.cpp``,
+  ``src/Kubo_solver_FFT_postProcess.cpp``);
+* :func:`jackson_1d`, :func:`jackson_2d`, :func:`dos_fftgrid`,
+  :func:`greenwood_fftgrid`, :func:`bastin_fftgrid` -- the Jackson kernels
+  and the FFT-grid post-processors.
+
+Every call runs on the GPU through ``liblinqt_amd.so``; nothing falls back to
+the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (LQ_ALGO_DIRECT, LQ_ALGO_DOUBLING, LQ_FILL_GENERAL, LQ_FILL_HERMITIAN_UPPER,
+                   LQ_KUBO_BASTIN, LQ_KUBO_GREENWOOD, LQ_PREC_FP64, LinqtError)
+
+__all__ = [
+    "Context", "SparseMatrix", "DeviceStates", "spectral_moments", "correlation_expansion_moments", "kubo_fft", "kubo_post",
+    "jackson_1d", "jackson_2d", "dos_fftgrid", "greenwood_fftgrid", "bastin_fftgrid", "LinqtError",
+    "LQ_FILL_HERMITIAN_UPPER", "LQ_FILL_GENERAL", "LQ_KUBO_GREENWOOD", "LQ_KUBO_BASTIN", "LQ_ALGO_DIRECT",
+    "LQ_ALGO_DOUBLING", "LQ_PREC_FP64",
+]
+
+
+def _dp(a: np.ndarray):
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a: np.ndarray):
+    assert a.flags.c_contiguous and a.dtype == np.int32
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = _lib.load().lq_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+class Context:
+    """One GPU, one HIP stream (``lq_ctx``)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _lib.call("lq_ctx_create", device, C.byref(h))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            _lib.load().lq_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self):
+        _lib.call("lq_ctx_synchronize", self.h)
+
+    def set_timing(self, on: bool = True):
+        _lib.call("lq_ctx_set_timing", self.h, 1 if on else 0)
+
+    def reset_timing(self):
+        _lib.call("lq_ctx_reset_timing", self.h)
+
+    def timing(self) -> dict:
+        t = _lib.lq_timing()
+        _lib.call("lq_ctx_get_timing", self.h, C.byref(t))
+        return {name: getattr(t, name) for name, _ in t._fields_}
+
+    # device buffers (for lq_spmv)
+    def alloc(self, nbytes: int) -> C.c_void_p:
+        p = C.c_void_p()
+        _lib.call("lq_malloc", self.h, nbytes, C.byref(p))
+        return p
+
+    def free(self, p: C.c_void_p):
+        _lib.call("lq_free", self.h, p)
+
+    def upload(self, p: C.c_void_p, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        _lib.call("lq_memcpy_h2d", self.h, p, a.ctypes.data_as(C.c_void_p), a.nbytes)
+
+    def download(self, p: C.c_void_p, a: np.ndarray):
+        assert a.flags.c_contiguous
+        _lib.call("lq_memcpy_d2h", self.h, a.ctypes.data_as(C.c_void_p), p, a.nbytes)
+
+
+class SparseMatrix:
+    """``SparseMatrixType``: a device operator built from host CSR arrays.
+
+    ``fill=LQ_FILL_HERMITIAN_UPPER`` (the reference's only descriptor) treats
+    the arrays as the upper triangle of a Hermitian matrix.
+    """
+
+    def __init__(self, ctx: Context, n, rowptr, col, val, fill: int = LQ_FILL_HERMITIAN_UPPER, id: str = ""):
+        self.ctx = ctx
+        self.n = int(n)
+        self.id = id
+        rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.complex128)
+        h = C.c_void_p()
+        _lib.call("lq_csr_create", ctx.h, self.n, int(col.shape[0]), _ip(rowptr), _ip(col),
+                  _dp(val.view(np.float64)), fill, C.byref(h))
+        self.h = h
+
+    @classmethod
+    def from_csr(cls, ctx: Context, m, fill: int = LQ_FILL_HERMITIAN_UPPER, id: str = ""):
+        return cls(ctx, m.n, m.rowptr, m.col, m.val, fill=fill, id=id)
+
+    @classmethod
+    def from_file(cls, ctx: Context, path: str, id: str = ""):
+        """``SparseMatrixBuilder::BuildOPFromCSRFile``."""
+        from .io import read_csr
+
+        return cls.from_csr(ctx, read_csr(path), id=id)
+
+    def isIdentity(self) -> bool:
+        return self.id == "1"
+
+    def close(self):
+        # a closed context already released the device; nothing to free then
+        if getattr(self, "h", None) is not None and self.h.value and self.ctx.h is not None:
+            _lib.load().lq_csr_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def Rescale(self, a, b):
+        """``SparseMatrixType::Rescale``: A <- a*A + b*I."""
+        a, b = complex(a), complex(b)
+        _lib.call("lq_csr_rescale", self.h, a.real, a.imag, b.real, b.imag)
+        return self
+
+    def info(self):
+        n, nnz, st = C.c_int(), C.c_longlong(), C.c_longlong()
+        _lib.call("lq_csr_info", self.h, C.byref(n), C.byref(nnz), C.byref(st))
+        return {"n": n.value, "nnz": nnz.value, "stored": st.value}
+
+    def Multiply(self, a, x: np.ndarray, b=0.0, y: np.ndarray | None = None) -> np.ndarray:
+        """``SparseMatrixType::Multiply(a, x, b, y)`` on host arrays.
+
+        x, y are (n,) or (n, k) complex; returns the new y."""
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        k = 1 if x.ndim == 1 else x.shape[1]
+        y = np.zeros_like(x) if y is None else np.ascontiguousarray(y, dtype=np.complex128).copy()
+        px = self.ctx.alloc(x.nbytes)
+        py = self.ctx.alloc(y.nbytes)
+        try:
+            self.ctx.upload(px, x)
+            self.ctx.upload(py, y)
+            a, b = complex(a), complex(b)
+            _lib.call("lq_spmv", self.h, a.real, a.imag, px, b.real, b.imag, py, k)
+            self.ctx.download(py, y)
+        finally:
+            self.ctx.free(px)
+            self.ctx.free(py)
+        return y
+
+
+class DeviceStates:
+    """R states of length n resident in device memory (state-major complex128).
+
+    Passing one to a solver keeps the inputs in HBM: no host transfer happens
+    inside the solve."""
+
+    def __init__(self, ctx: Context, phi: np.ndarray):
+        phi = np.ascontiguousarray(np.atleast_2d(phi), dtype=np.complex128)
+        self.ctx = ctx
+        self.R, self.n = phi.shape
+        self.ptr = ctx.alloc(phi.nbytes)
+        ctx.upload(self.ptr, phi)
+
+    def close(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value and self.ctx.h is not None:
+            self.ctx.free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _states(phi):
+    """Returns (double pointer, R) for host arrays or DeviceStates."""
+    if isinstance(phi, DeviceStates):
+        return C.cast(phi.ptr, C.POINTER(C.c_double)), phi.R, None
+    arr = np.ascontiguousarray(np.atleast_2d(phi), dtype=np.complex128)
+    return _dp(arr.view(np.float64)), arr.shape[0], arr
+
+
+def _h(op):
+    return None if op is None else op.h
+
+
+def spectral_moments(H: SparseMatrix, OP, phi: np.ndarray, M: int, algo: int = LQ_ALGO_DIRECT,
+                     mu: np.ndarray | None = None) -> np.ndarray:
+    """``chebyshev::SpectralMoments``: mu_m += sum_r (2/R)(1/2)^{delta_m0} <phi_r|T_m(H) OP|phi_r>.
+
+    OP None (or an operator with id "1") is the identity."""
+    if OP is not None and OP.isIdentity():
+        OP = None
+    pp, R, keep = _states(phi)
+    mu = np.zeros(M, dtype=np.complex128) if mu is None else np.array(mu, dtype=np.complex128)
+    _lib.call("lq_spectral_moments", H.h, _h(OP), pp, R, M, algo, _dp(mu.view(np.float64)))
+    return mu
+
+
+def correlation_expansion_moments(H: SparseMatrix, OPL, OPR, phi: np.ndarray, M0: int, M1: int | None = None,
+                                  batch: int = 0, precision: int = LQ_PREC_FP64, mu=None) -> np.ndarray:
+    """``chebyshev::CorrelationExpansionMoments``.
+
+    OPR may be one operator or a list (sharing the left recursion); returns
+    (M0, M1) or (len(OPR), M0, M1) complex tables."""
+    M1 = M0 if M1 is None else M1
+    single = not isinstance(OPR, (list, tuple))
+    oprs = [OPR] if single else list(OPR)
+    arr = (C.c_void_p * len(oprs))(*[None if o is None else o.h.value for o in oprs])
+    pp, R, keep = _states(phi)
+    if mu is None:
+        out = np.zeros((len(oprs), M0, M1), dtype=np.complex128)
+    else:
+        out = np.array(mu, dtype=np.complex128).reshape(len(oprs), M0, M1).copy()
+    _lib.call("lq_correlation_moments", H.h, _h(OPL), len(oprs), C.cast(arr, C.POINTER(C.c_void_p)),
+              pp, R, M0, M1, batch, precision, _dp(out.view(np.float64)))
+    return out[0] if single else out
+
+
+def kubo_fft(H: SparseMatrix, OPL, OPR, phi: np.ndarray, M: int, nump: int, formula: int,
+             num_sections: int = 1, precision: int = LQ_PREC_FP64) -> np.ndarray:
+    """``Kubo_solver_FFT::compute``: the mean over states of r_data, (2*nump,) per operator."""
+    single = not isinstance(OPR, (list, tuple))
+    oprs = [OPR] if single else list(OPR)
+    arr = (C.c_void_p * len(oprs))(*[None if o is None else o.h.value for o in oprs])
+    pp, R, keep = _states(phi)
+    fd = np.zeros((len(oprs), 2 * nump), dtype=np.complex128)
+    _lib.call("lq_kubo_fft", H.h, _h(OPL), len(oprs), C.cast(arr, C.POINTER(C.c_void_p)),
+              pp, R, M, num_sections, nump, formula, precision, _dp(fd.view(np.float64)))
+    return fd[0] if single else fd
+
+
+def kubo_post(ctx: Context, formula: int, final_data: np.ndarray, n: int, W: float, Ec: float):
+    """``Kubo_solver_FFT_postProcess``: returns (E, sigma, integrand)."""
+    fd = np.ascontiguousarray(final_data, dtype=np.complex128)
+    nump = fd.shape[0] // 2
+    E, s, g = np.zeros(nump), np.zeros(nump), np.zeros(nump)
+    _lib.call("lq_kubo_post", ctx.h, formula, _dp(fd.view(np.float64)), nump, n, W, Ec, _dp(E), _dp(s), _dp(g))
+    return E, s, g
+
+
+def jackson_1d(ctx: Context, mu: np.ndarray, W: float, broad_meV: float) -> np.ndarray:
+    mu = np.array(mu, dtype=np.complex128)
+    m = C.c_int(0)
+    _lib.call("lq_jackson_1d", ctx.h, _dp(mu.view(np.float64)), mu.shape[0], W, broad_meV, C.byref(m))
+    return mu[: m.value].copy()
+
+
+def jackson_2d(ctx: Context, mu: np.ndarray, W: float, b0: float, b1: float) -> np.ndarray:
+    mu = np.array(mu, dtype=np.complex128)
+    M0, M1 = mu.shape
+    o0, o1 = C.c_int(0), C.c_int(0)
+    _lib.call("lq_jackson_2d", ctx.h, _dp(mu.view(np.float64)), M0, M1, W, b0, b1, C.byref(o0), C.byref(o1))
+    return mu.reshape(-1)[: o0.value * o1.value].reshape(o0.value, o1.value).copy()
+
+
+def dos_fftgrid(ctx: Context, mu: np.ndarray, n: int, W: float, Ec: float, num_div: int):
+    mu = np.ascontiguousarray(mu, dtype=np.complex128)
+    E, rho = np.zeros(num_div), np.zeros(num_div)
+    _lib.call("lq_dos_fftgrid", ctx.h, _dp(mu.view(np.float64)), mu.shape[0], n, W, Ec, num_div, _dp(E), _dp(rho))
+    return E, rho
+
+
+def greenwood_fftgrid(ctx: Context, mu: np.ndarray, n: int, W: float, Ec: float, num_div: int):
+    mu = np.ascontiguousarray(mu, dtype=np.complex128)
+    E, s = np.zeros(num_div - 1), np.zeros(num_div - 1)
+    _lib.call("lq_greenwood_fftgrid", ctx.h, _dp(mu.view(np.float64)), mu.shape[0], mu.shape[1], n, W, Ec, num_div,
+              _dp(E), _dp(s))
+    return E, s
+
+
+def bastin_fftgrid(ctx: Context, mu: np.ndarray, n: int, W: float, Ec: float, num_div: int):
+    mu = np.ascontiguousarray(mu, dtype=np.complex128)
+    E, s = np.zeros(num_div - 1), np.zeros(num_div - 1)
+    _lib.call("lq_bastin_fftgrid", ctx.h, _dp(mu.view(np.float64)), mu.shape[0], mu.shape[1], n, W, Ec, num_div,
+              _dp(E), _dp(s))
+    return E, s

@@ -1,0 +1,117 @@
+"""ctypes binding of ``liblinqt_amd.so`` (the C ABI in ``include/linqt_amd.h``).
+
+There is no CPU fallback: if the library or a GPU is missing, every entry
+point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblinqt_amd.so")
+
+LQ_OK = 0
+LQ_ERR_ARG = -1
+LQ_ERR_HIP = -2
+LQ_ERR_NOMEM = -3
+LQ_ERR_UNSUPPORTED = -4
+LQ_ERR_NODEV = -5
+
+LQ_FILL_HERMITIAN_UPPER = 0
+LQ_FILL_GENERAL = 1
+LQ_KUBO_GREENWOOD = 0
+LQ_KUBO_BASTIN = 1
+LQ_ALGO_DIRECT = 0
+LQ_ALGO_DOUBLING = 1
+LQ_PREC_FP64 = 0
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+
+class lq_timing(C.Structure):
+    _fields_ = [
+        ("spmv_ms", C.c_double),
+        ("contract_ms", C.c_double),
+        ("post_ms", C.c_double),
+        ("spmv_bytes", C.c_double),
+        ("contract_flops", C.c_double),
+        ("spmv_launches", C.c_longlong),
+        ("contract_launches", C.c_longlong),
+    ]
+
+
+# name -> (restype, argtypes); every function returns int unless noted
+_SIGS = {
+    "lq_last_error": (C.c_char_p, []),
+    "lq_abi_version": (C.c_int, []),
+    "lq_device_count": (C.c_int, [_ip]),
+    "lq_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "lq_ctx_destroy": (C.c_int, [_vp]),
+    "lq_ctx_synchronize": (C.c_int, [_vp]),
+    "lq_ctx_set_timing": (C.c_int, [_vp, C.c_int]),
+    "lq_ctx_reset_timing": (C.c_int, [_vp]),
+    "lq_ctx_get_timing": (C.c_int, [_vp, C.POINTER(lq_timing)]),
+    "lq_malloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "lq_free": (C.c_int, [_vp, _vp]),
+    "lq_memcpy_h2d": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "lq_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "lq_csr_create": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, _dp, C.c_int, C.POINTER(_vp)]),
+    "lq_csr_destroy": (C.c_int, [_vp]),
+    "lq_csr_rescale": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "lq_csr_info": (C.c_int, [_vp, _ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "lq_spmv": (C.c_int, [_vp, C.c_double, C.c_double, _vp, C.c_double, C.c_double, _vp, C.c_int]),
+    "lq_compute_mom_table": (C.c_int, [_vp, _vp, C.c_longlong, C.c_int, _vp, C.c_longlong, C.c_int, C.c_longlong,
+                                       _vp, C.c_int]),
+    "lq_spectral_moments": (C.c_int, [_vp, _vp, _dp, C.c_int, C.c_int, C.c_int, _dp]),
+    "lq_correlation_moments": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(_vp), _dp, C.c_int, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, _dp]),
+    "lq_kubo_fft": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(_vp), _dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                              C.c_int, C.c_int, _dp]),
+    "lq_kubo_post": (C.c_int, [_vp, C.c_int, _dp, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp]),
+    "lq_jackson_1d": (C.c_int, [_vp, _dp, C.c_int, C.c_double, C.c_double, _ip]),
+    "lq_jackson_2d": (C.c_int, [_vp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _ip, _ip]),
+    "lq_dos_fftgrid": (C.c_int, [_vp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]),
+    "lq_greenwood_fftgrid": (C.c_int, [_vp, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                       _dp, _dp]),
+    "lq_bastin_fftgrid": (C.c_int, [_vp, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                    _dp, _dp]),
+}
+
+_lib = None
+
+
+class LinqtError(RuntimeError):
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed ({code}): {msg}")
+        self.code = code
+
+
+def load():
+    """Loads the library (no GPU call happens here)."""
+    global _lib
+    if _lib is None:
+        if not os.path.isfile(LIB_PATH):
+            raise RuntimeError(f"liblinqt_amd.so is not built ({LIB_PATH}); run `python -m linqt_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def call(name: str, *args) -> int:
+    L = load()
+    rc = getattr(L, name)(*args)
+    if rc != LQ_OK:
+        msg = L.lq_last_error()
+        raise LinqtError(name, rc, msg.decode() if msg else "")
+    return rc

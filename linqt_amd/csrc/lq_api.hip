@@ -1,0 +1,1057 @@
+// lq_api.hip -- the C ABI (include/linqt_amd.h): contexts, operators and the
+// moment pipelines built from the kernels in lq_spmv / lq_contract / lq_post.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "lq_internal.h"
+
+namespace lq {
+
+namespace {
+thread_local std::string g_last_error;
+
+constexpr double kPi = 3.14159265358979323846;
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+inline double2 d2(double x, double y) { return make_double2(x, y); }
+
+}  // namespace
+
+void set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+int ctx_buf(lq_ctx ctx, const char *name, size_t bytes, void **p) {
+  auto &b = ctx->bufs[name];
+  if (bytes == 0) bytes = 16;
+  if (b.bytes < bytes) {
+    if (b.p) {
+      LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      LQ_HIP_CHECK(hipFree(b.p));
+      b.p = nullptr;
+      b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      b.p = nullptr;
+      set_error("device allocation of %zu bytes for '%s' failed: %s", bytes, name, hipGetErrorString(e));
+      return LQ_ERR_NOMEM;
+    }
+    b.bytes = bytes;
+  }
+  *p = b.p;
+  return LQ_OK;
+}
+
+int time_begin(lq_ctx ctx, int cat, hipEvent_t *ev) {
+  (void)cat;
+  *ev = nullptr;
+  if (!ctx->timing) return LQ_OK;
+  if (ctx->event_pool.empty()) {
+    hipEvent_t e;
+    LQ_HIP_CHECK(hipEventCreate(&e));
+    ctx->event_pool.push_back(e);
+  }
+  *ev = ctx->event_pool.back();
+  ctx->event_pool.pop_back();
+  LQ_HIP_CHECK(hipEventRecord(*ev, ctx->stream));
+  return LQ_OK;
+}
+
+int time_end(lq_ctx ctx, int cat, hipEvent_t ev, double amount) {
+  if (!ctx->timing || !ev) return LQ_OK;
+  if (ctx->event_pool.empty()) {
+    hipEvent_t e;
+    LQ_HIP_CHECK(hipEventCreate(&e));
+    ctx->event_pool.push_back(e);
+  }
+  hipEvent_t b = ctx->event_pool.back();
+  ctx->event_pool.pop_back();
+  LQ_HIP_CHECK(hipEventRecord(b, ctx->stream));
+  ctx->pending.push_back({cat, ev, b, amount});
+  return LQ_OK;
+}
+
+int collect_timing(lq_ctx ctx) {
+  if (ctx->pending.empty()) return LQ_OK;
+  LQ_HIP_CHECK(hipEventSynchronize(ctx->pending.back().b));
+  for (auto &p : ctx->pending) {
+    float ms = 0.f;
+    LQ_HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+    switch (p.cat) {
+      case kCatSpmv:
+        ctx->tsum.spmv_ms += ms;
+        ctx->tsum.spmv_bytes += p.bytes_or_flops;
+        ctx->tsum.spmv_launches++;
+        break;
+      case kCatContract:
+        ctx->tsum.contract_ms += ms;
+        ctx->tsum.contract_flops += p.bytes_or_flops;
+        ctx->tsum.contract_launches++;
+        break;
+      default:
+        ctx->tsum.post_ms += ms;
+        break;
+    }
+    ctx->event_pool.push_back(p.a);
+    ctx->event_pool.push_back(p.b);
+  }
+  ctx->pending.clear();
+  return LQ_OK;
+}
+
+namespace {
+
+// ---- operators ------------------------------------------------------------
+
+void free_dev(SellDev &d) {
+  if (d.sptr) (void)hipFree(d.sptr);
+  if (d.col) (void)hipFree(d.col);
+  if (d.val) (void)hipFree(d.val);
+  d = SellDev{};
+}
+
+// Lays the completed CSR out as sliced ELL (64-row slices, column-major
+// inside a slice) and uploads it.
+int upload_sell(lq_csr A) {
+  const int n = A->n;
+  const int nslices = (n + kSlice - 1) / kSlice;
+  std::vector<int> sptr(nslices + 1, 0);
+  for (int s = 0; s < nslices; ++s) {
+    int w = 0;
+    for (int r = 0; r < kSlice; ++r) {
+      const int i = s * kSlice + r;
+      if (i < n) w = std::max(w, A->rowptr[i + 1] - A->rowptr[i]);
+    }
+    const long long next = (long long)sptr[s] + (long long)w * kSlice;
+    if (next > 0x7fffffffLL) {
+      set_error("operator too large for 32-bit sliced-ELL offsets");
+      return LQ_ERR_ARG;
+    }
+    sptr[s + 1] = (int)next;
+  }
+  const size_t stored = (size_t)sptr[nslices];
+  std::vector<int> col(stored > 0 ? stored : 1);
+  std::vector<double2> val(stored > 0 ? stored : 1);
+  for (int s = 0; s < nslices; ++s) {
+    const int w = (sptr[s + 1] - sptr[s]) / kSlice;
+    for (int r = 0; r < kSlice; ++r) {
+      const int i = s * kSlice + r;
+      const int len = i < n ? A->rowptr[i + 1] - A->rowptr[i] : 0;
+      for (int j = 0; j < w; ++j) {
+        const size_t p = (size_t)sptr[s] + (size_t)j * kSlice + r;
+        if (j < len) {
+          const int q = A->rowptr[i] + j;
+          col[p] = A->col[q];
+          val[p] = d2(A->val[q].real(), A->val[q].imag());
+        } else {
+          col[p] = i < n ? i : 0;
+          val[p] = d2(0.0, 0.0);
+        }
+      }
+    }
+  }
+  free_dev(A->dev);
+  SellDev d;
+  d.n = n;
+  d.nslices = nslices;
+  d.stored = (long long)stored;
+  LQ_HIP_CHECK(hipMalloc(&d.sptr, sizeof(int) * sptr.size()));
+  LQ_HIP_CHECK(hipMalloc(&d.col, sizeof(int) * col.size()));
+  LQ_HIP_CHECK(hipMalloc(&d.val, sizeof(double2) * val.size()));
+  LQ_HIP_CHECK(hipMemcpy(d.sptr, sptr.data(), sizeof(int) * sptr.size(), hipMemcpyHostToDevice));
+  LQ_HIP_CHECK(hipMemcpy(d.col, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
+  LQ_HIP_CHECK(hipMemcpy(d.val, val.data(), sizeof(double2) * val.size(), hipMemcpyHostToDevice));
+  A->dev = d;
+  return LQ_OK;
+}
+
+struct Entry {
+  int row, col;
+  std::complex<double> v;
+  signed char kind;
+};
+
+// Sorts entries by (row, col) keeping input order among equal keys, and
+// fills the host CSR of A.
+void set_entries(lq_csr A, std::vector<Entry> &e) {
+  std::stable_sort(e.begin(), e.end(), [](const Entry &a, const Entry &b) {
+    return a.row != b.row ? a.row < b.row : a.col < b.col;
+  });
+  A->rowptr.assign(A->n + 1, 0);
+  A->col.resize(e.size());
+  A->val.resize(e.size());
+  A->kind.resize(e.size());
+  for (size_t q = 0; q < e.size(); ++q) {
+    A->rowptr[e[q].row + 1]++;
+    A->col[q] = e[q].col;
+    A->val[q] = e[q].v;
+    A->kind[q] = e[q].kind;
+  }
+  for (int i = 0; i < A->n; ++i) A->rowptr[i + 1] += A->rowptr[i];
+}
+
+std::vector<Entry> get_entries(const lq_csr A) {
+  std::vector<Entry> e(A->col.size());
+  for (int i = 0; i < A->n; ++i)
+    for (int q = A->rowptr[i]; q < A->rowptr[i + 1]; ++q) e[q] = {i, A->col[q], A->val[q], A->kind[q]};
+  return e;
+}
+
+int check_op(lq_csr A, const char *what) {
+  if (!A || !A->ctx) {
+    set_error("%s: null operator", what);
+    return LQ_ERR_ARG;
+  }
+  return LQ_OK;
+}
+
+long long spmv_bytes(const lq_csr A, int k, bool with_z) {
+  return 20LL * (long long)A->col.size() + 4LL * (A->n + 1) + (with_z ? 48LL : 32LL) * k * (long long)A->n;
+}
+
+// One timed SpMM launch.
+int spmm(lq_ctx ctx, lq_csr A, double2 alpha, double2 beta, const VecSet *sets, int nsets, int dot_mode = kDotNone,
+         const double2 *phi = nullptr, double2 *partial = nullptr) {
+  hipEvent_t ev;
+  LQ_TRY(time_begin(ctx, kCatSpmv, &ev));
+  LQ_TRY(launch_spmm(ctx->stream, A->dev, alpha, beta, sets, nsets, dot_mode, phi, partial, nullptr));
+  int k = 0;
+  for (int s = 0; s < nsets; ++s) k += sets[s].k;
+  return time_end(ctx, kCatSpmv, ev, (double)spmv_bytes(A, k, sets[0].z != nullptr));
+}
+
+bool is_device_ptr(const void *p, int device) {
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice && attr.device == device;
+}
+
+// State-major phi (host, or device memory of ctx's GPU) -> device [n][g]
+// interleaved block of states r0..r0+g-1.
+int upload_states(lq_ctx ctx, const double *phi, int n, int r0, int g, double2 *dst) {
+  if (is_device_ptr(phi, ctx->device)) return launch_interleave(ctx->stream, (const double2 *)phi, n, r0, g, dst);
+  std::vector<double2> h((size_t)n * g);
+  for (int v = 0; v < g; ++v) {
+    const double *src = phi + 2 * ((size_t)(r0 + v) * n);
+    for (int i = 0; i < n; ++i) h[(size_t)i * g + v] = d2(src[2 * i], src[2 * i + 1]);
+  }
+  LQ_HIP_CHECK(hipMemcpyAsync(dst, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return LQ_OK;
+}
+
+size_t free_device_bytes() {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return fr;
+}
+
+// ---- the 2D pipeline --------------------------------------------------------
+// For one group of g states (interleaved on the device) it stores
+//   L_m = T_m(H~) OPL phi,  m < ML      (left recursion, src/chebyshev_moments.cpp:5-22)
+//   R^o_n = OPR_o T_n(H~) phi, n < MR   (right recursion + Multiply, src/chebyshev_vectors.cpp:24-51)
+// and accumulates raw_o += L^H R^o (K = n*g).
+struct Pipe2D {
+  lq_ctx ctx;
+  lq_csr H, OPL;
+  int nopr;
+  const lq_csr *OPR;
+  int n, ML, MR;
+  double2 *raw;  // nopr tables of ML x MR (ld MR)
+};
+
+int pipe2d_group(Pipe2D &P, const double *phi, int r0, int g, bool accumulate) {
+  lq_ctx ctx = P.ctx;
+  const int n = P.n;
+  const size_t vec = (size_t)n * g;
+  double2 *phi_d, *Ls, *Rw, *Rs, *work;
+  LQ_TRY(ctx_buf(ctx, "phi", vec * sizeof(double2), (void **)&phi_d));
+  LQ_TRY(ctx_buf(ctx, "Lstore", vec * (size_t)P.ML * sizeof(double2), (void **)&Ls));
+  LQ_TRY(ctx_buf(ctx, "Rwork", 2 * vec * sizeof(double2), (void **)&Rw));
+  LQ_TRY(ctx_buf(ctx, "Rstore", vec * (size_t)P.MR * P.nopr * sizeof(double2), (void **)&Rs));
+  size_t wbytes = 0;
+  int nsplit = 1;
+  LQ_TRY(contract_workspace_bytes(P.ML, P.MR, (long long)vec, &wbytes, &nsplit));
+  LQ_TRY(ctx_buf(ctx, "cwork", wbytes, (void **)&work));
+  LQ_TRY(upload_states(ctx, phi, n, r0, g, phi_d));
+
+  const double2 one = d2(1, 0), zero = d2(0, 0), two = d2(2, 0), mone = d2(-1, 0);
+  double2 *Rcur = Rw, *Rprev = Rw + vec;  // Rcur = T_n, Rprev = T_{n-1}
+  auto L = [&](int m) { return Ls + (size_t)m * vec; };
+  auto RS = [&](int o, int m) { return Rs + ((size_t)o * P.MR + m) * vec; };
+
+  // T0 of the left recursion: OPL * phi
+  if (P.OPL) {
+    VecSet s{phi_d, nullptr, L(0), g};
+    LQ_TRY(spmm(ctx, P.OPL, one, zero, &s, 1));
+  } else {
+    LQ_TRY(launch_copy(ctx->stream, L(0), phi_d, vec));
+  }
+  LQ_TRY(launch_copy(ctx->stream, Rprev, phi_d, vec));  // right T0 = phi
+  auto apply_opr = [&](const double2 *Tn, int m) -> int {
+    for (int o = 0; o < P.nopr; ++o) {
+      if (P.OPR[o]) {
+        VecSet s{Tn, nullptr, RS(o, m), g};
+        LQ_TRY(spmm(ctx, P.OPR[o], one, zero, &s, 1));
+      } else {
+        LQ_TRY(launch_copy(ctx->stream, RS(o, m), Tn, vec));
+      }
+    }
+    return LQ_OK;
+  };
+  if (P.MR > 0) LQ_TRY(apply_opr(Rprev, 0));
+  // T1 = H~ T0 on both sides
+  const int Mmax = std::max(P.ML, P.MR);
+  if (Mmax > 1) {
+    VecSet sets[2];
+    int ns = 0;
+    if (P.ML > 1) sets[ns++] = VecSet{L(0), nullptr, L(1), g};
+    if (P.MR > 1) sets[ns++] = VecSet{Rprev, nullptr, Rcur, g};
+    LQ_TRY(spmm(ctx, P.H, one, zero, sets, ns));
+    if (P.MR > 1) LQ_TRY(apply_opr(Rcur, 1));
+  }
+  // T_{m+1} = 2 H~ T_m - T_{m-1}
+  for (int m = 1; m + 1 < Mmax; ++m) {
+    VecSet sets[2];
+    int ns = 0;
+    if (m + 1 < P.ML) sets[ns++] = VecSet{L(m), L(m - 1), L(m + 1), g};
+    const bool rstep = m + 1 < P.MR;
+    if (rstep) sets[ns++] = VecSet{Rcur, Rprev, Rprev, g};
+    LQ_TRY(spmm(ctx, P.H, two, mone, sets, ns));
+    if (rstep) {
+      std::swap(Rcur, Rprev);
+      LQ_TRY(apply_opr(Rcur, m + 1));
+    }
+  }
+  // raw_o (+)= L^H R^o
+  for (int o = 0; o < P.nopr; ++o) {
+    hipEvent_t ev;
+    LQ_TRY(time_begin(ctx, kCatContract, &ev));
+    LQ_TRY(launch_contract_f64(ctx->stream, Ls, (long long)vec, P.ML, RS(o, 0), (long long)vec, P.MR, (long long)vec,
+                               work, nsplit));
+    LQ_TRY(time_end(ctx, kCatContract, ev, 8.0 * P.ML * (double)P.MR * (double)vec));
+    LQ_TRY(time_begin(ctx, kCatPost, &ev));
+    LQ_TRY(launch_contract_reduce(ctx->stream, work, nsplit, P.ML, P.MR, P.raw + (size_t)o * P.ML * P.MR, P.MR,
+                                  accumulate ? 1 : 0));
+    LQ_TRY(time_end(ctx, kCatPost, ev, 0));
+  }
+  return LQ_OK;
+}
+
+// States per group: as many as fit (vectors per SpMM row <= 64 in total).
+int choose_group(int R, int n, int ML, int MR, int nopr) {
+  const size_t per_state = ((size_t)ML + (size_t)MR * nopr + 3) * (size_t)n * sizeof(double2);
+  size_t budget = free_device_bytes();
+  budget = budget > ((size_t)4 << 30) ? budget - ((size_t)4 << 30) : budget / 2;  // keep head room
+  int g = (int)std::min<size_t>((size_t)R, per_state ? budget / per_state : (size_t)R);
+  g = std::max(1, std::min(g, 32));
+  const int ngroups = (R + g - 1) / g;
+  return (R + ngroups - 1) / ngroups;
+}
+
+int run_pipe2d(Pipe2D &P, const double *phi, int R) {
+  const int g = choose_group(R, P.n, P.ML, P.MR, P.nopr);
+  for (int r0 = 0; r0 < R; r0 += g) {
+    const int gg = std::min(g, R - r0);
+    LQ_TRY(pipe2d_group(P, phi, r0, gg, r0 > 0));
+  }
+  return LQ_OK;
+}
+
+int ord_of(int p, int B) { return (p / B) * (B - 1) + (p % B); }
+
+// Section-masked copy of an operator: entries whose row and column fall in
+// different sections are dropped, which is what MultiplySliced computes with
+// num_sections > 1 (src/chebyshev_vectors.cpp:24-51).
+int masked_copy(lq_csr A, int ns, lq_csr *out) {
+  const int n = A->n, ss = n / ns;
+  auto sec = [&](int i) { return std::min(i / ss, ns - 1); };
+  std::vector<int> rp(n + 1, 0), cl;
+  std::vector<double> vl;
+  for (int i = 0; i < n; ++i) {
+    for (int q = A->rowptr[i]; q < A->rowptr[i + 1]; ++q)
+      if (sec(A->col[q]) == sec(i)) {
+        cl.push_back(A->col[q]);
+        vl.push_back(A->val[q].real());
+        vl.push_back(A->val[q].imag());
+      }
+    rp[i + 1] = (int)cl.size();
+  }
+  return lq_csr_create(A->ctx, n, (int)cl.size(), rp.data(), cl.data(), vl.data(), LQ_FILL_GENERAL, out);
+}
+
+}  // namespace
+}  // namespace lq
+
+using namespace lq;
+
+// ============================================================================
+extern "C" {
+
+const char *lq_last_error(void) { return g_last_error.c_str(); }
+
+int lq_abi_version(void) { return LQ_ABI_VERSION; }
+
+int lq_device_count(int *count) {
+  if (!count) return LQ_ERR_ARG;
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *count = 0;
+    set_error("hipGetDeviceCount: %s", hipGetErrorString(e));
+    return LQ_ERR_NODEV;
+  }
+  return LQ_OK;
+}
+
+int lq_ctx_create(int device, lq_ctx *out) {
+  if (!out) {
+    set_error("lq_ctx_create: null output");
+    return LQ_ERR_ARG;
+  }
+  *out = nullptr;
+  int count = 0;
+  LQ_TRY(lq_device_count(&count));
+  if (device < 0 || device >= count) {
+    set_error("lq_ctx_create: device %d not present (%d GPUs)", device, count);
+    return LQ_ERR_NODEV;
+  }
+  DeviceGuard g(device);
+  lq_ctx c = new lq_ctx_s();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    set_error("hipStreamCreate: %s", hipGetErrorString(e));
+    return LQ_ERR_HIP;
+  }
+  *out = c;
+  return LQ_OK;
+}
+
+int lq_ctx_destroy(lq_ctx ctx) {
+  if (!ctx) return LQ_OK;
+  DeviceGuard g(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &kv : ctx->bufs)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  for (auto &p : ctx->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return LQ_OK;
+}
+
+int lq_ctx_synchronize(lq_ctx ctx) {
+  if (!ctx) return LQ_ERR_ARG;
+  DeviceGuard g(ctx->device);
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return collect_timing(ctx);
+}
+
+int lq_ctx_set_timing(lq_ctx ctx, int enabled) {
+  if (!ctx) return LQ_ERR_ARG;
+  ctx->timing = enabled != 0;
+  return LQ_OK;
+}
+
+int lq_ctx_reset_timing(lq_ctx ctx) {
+  if (!ctx) return LQ_ERR_ARG;
+  DeviceGuard g(ctx->device);
+  LQ_TRY(collect_timing(ctx));
+  ctx->tsum = lq_timing{};
+  return LQ_OK;
+}
+
+int lq_ctx_get_timing(lq_ctx ctx, lq_timing *out) {
+  if (!ctx || !out) return LQ_ERR_ARG;
+  DeviceGuard g(ctx->device);
+  LQ_TRY(collect_timing(ctx));
+  *out = ctx->tsum;
+  return LQ_OK;
+}
+
+int lq_malloc(lq_ctx ctx, size_t bytes, void **dptr) {
+  if (!ctx || !dptr) return LQ_ERR_ARG;
+  DeviceGuard g(ctx->device);
+  hipError_t e = hipMalloc(dptr, bytes ? bytes : 16);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *dptr = nullptr;
+    set_error("lq_malloc(%zu): %s", bytes, hipGetErrorString(e));
+    return LQ_ERR_NOMEM;
+  }
+  return LQ_OK;
+}
+
+int lq_free(lq_ctx ctx, void *dptr) {
+  if (!ctx) return LQ_ERR_ARG;
+  if (!dptr) return LQ_OK;
+  DeviceGuard g(ctx->device);
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  LQ_HIP_CHECK(hipFree(dptr));
+  return LQ_OK;
+}
+
+int lq_memcpy_h2d(lq_ctx ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return LQ_ERR_ARG;
+  DeviceGuard g(ctx->device);
+  LQ_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return LQ_OK;
+}
+
+int lq_memcpy_d2h(lq_ctx ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return LQ_ERR_ARG;
+  DeviceGuard g(ctx->device);
+  LQ_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return LQ_OK;
+}
+
+// ---- operators ---------------------------------------------------------------
+int lq_csr_create(lq_ctx ctx, int n, int nnz, const int *rowptr, const int *col, const double *val, int fill,
+                  lq_csr *out) {
+  if (!ctx || !out || n <= 0 || nnz < 0 || !rowptr || (nnz > 0 && (!col || !val))) {
+    set_error("lq_csr_create: invalid arguments (n=%d nnz=%d)", n, nnz);
+    return LQ_ERR_ARG;
+  }
+  if (fill != LQ_FILL_HERMITIAN_UPPER && fill != LQ_FILL_GENERAL) {
+    set_error("lq_csr_create: unknown fill mode %d", fill);
+    return LQ_ERR_ARG;
+  }
+  *out = nullptr;
+  if (rowptr[0] != 0 || rowptr[n] != nnz) {
+    set_error("lq_csr_create: rowptr[0]=%d rowptr[n]=%d, expected 0 and nnz=%d", rowptr[0], rowptr[n], nnz);
+    return LQ_ERR_ARG;
+  }
+  std::vector<Entry> e;
+  e.reserve((size_t)nnz * (fill == LQ_FILL_HERMITIAN_UPPER ? 2 : 1));
+  for (int i = 0; i < n; ++i) {
+    if (rowptr[i + 1] < rowptr[i]) {
+      set_error("lq_csr_create: rowptr decreases at row %d", i);
+      return LQ_ERR_ARG;
+    }
+    for (int q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+      const int j = col[q];
+      if (j < 0 || j >= n) {
+        set_error("lq_csr_create: column %d out of range in row %d", j, i);
+        return LQ_ERR_ARG;
+      }
+      const std::complex<double> v(val[2 * (size_t)q], val[2 * (size_t)q + 1]);
+      if (fill == LQ_FILL_GENERAL) {
+        e.push_back({i, j, v, (signed char)(j == i ? 2 : 0)});
+      } else {
+        if (j < i) continue;  // MKL's upper-fill descriptor ignores the lower triangle
+        e.push_back({i, j, v, (signed char)(j == i ? 2 : 0)});
+        if (j != i) e.push_back({j, i, std::conj(v), 1});
+      }
+    }
+  }
+  DeviceGuard g(ctx->device);
+  lq_csr A = new lq_csr_s();
+  A->ctx = ctx;
+  A->n = n;
+  A->fill = fill;
+  set_entries(A, e);
+  int rc = upload_sell(A);
+  if (rc != LQ_OK) {
+    free_dev(A->dev);
+    delete A;
+    return rc;
+  }
+  *out = A;
+  return LQ_OK;
+}
+
+int lq_csr_destroy(lq_csr A) {
+  if (!A) return LQ_OK;
+  DeviceGuard g(A->ctx->device);
+  (void)hipStreamSynchronize(A->ctx->stream);
+  free_dev(A->dev);
+  delete A;
+  return LQ_OK;
+}
+
+// MKL's add C = a*U + b*I on the stored (upper) entries; the mirrored
+// entries stay the conjugates of the scaled upper ones, and a row without a
+// stored diagonal gains one (src/mkl_sparse_matrix.cpp:73-97).
+int lq_csr_rescale(lq_csr A, double a_re, double a_im, double b_re, double b_im) {
+  LQ_TRY(check_op(A, "lq_csr_rescale"));
+  const std::complex<double> a(a_re, a_im), b(b_re, b_im);
+  std::vector<Entry> e = get_entries(A);
+  std::vector<char> has_diag(A->n, 0);
+  for (auto &x : e) {
+    if (x.kind == 0) {
+      x.v = a * x.v;
+    } else if (x.kind == 1) {
+      x.v = std::conj(a * std::conj(x.v));
+    } else {
+      x.v = a * x.v;
+      if (!has_diag[x.row]) {
+        x.v = x.v + b;
+        has_diag[x.row] = 1;
+      }
+    }
+  }
+  for (int i = 0; i < A->n; ++i)
+    if (!has_diag[i]) e.push_back({i, i, b, 2});
+  DeviceGuard g(A->ctx->device);
+  LQ_HIP_CHECK(hipStreamSynchronize(A->ctx->stream));
+  set_entries(A, e);
+  return upload_sell(A);
+}
+
+int lq_csr_info(lq_csr A, int *n, long long *nnz, long long *nnz_stored) {
+  LQ_TRY(check_op(A, "lq_csr_info"));
+  if (n) *n = A->n;
+  if (nnz) *nnz = (long long)A->col.size();
+  if (nnz_stored) *nnz_stored = A->dev.stored;
+  return LQ_OK;
+}
+
+int lq_spmv(lq_csr A, double a_re, double a_im, const double *x, double b_re, double b_im, double *y, int k) {
+  LQ_TRY(check_op(A, "lq_spmv"));
+  if (!x || !y || k <= 0 || k > 64 || x == y) {
+    set_error("lq_spmv: need distinct x, y and 1 <= k <= 64");
+    return LQ_ERR_ARG;
+  }
+  lq_ctx ctx = A->ctx;
+  DeviceGuard g(ctx->device);
+  const bool use_z = !(b_re == 0.0 && b_im == 0.0);
+  VecSet s{(const double2 *)x, use_z ? (const double2 *)y : nullptr, (double2 *)y, k};
+  return spmm(ctx, A, d2(a_re, a_im), d2(b_re, b_im), &s, 1);
+}
+
+int lq_compute_mom_table(lq_ctx ctx, const double *L, long long ldl, int ML, const double *R, long long ldr, int MR,
+                         long long K, double *sub, int ldsub) {
+  if (!ctx || !L || !R || !sub || ML <= 0 || MR <= 0 || K <= 0 || ldl < K || ldr < K || ldsub < MR) {
+    set_error("lq_compute_mom_table: invalid arguments");
+    return LQ_ERR_ARG;
+  }
+  DeviceGuard dg(ctx->device);
+  size_t wbytes = 0;
+  int nsplit = 1;
+  LQ_TRY(contract_workspace_bytes(ML, MR, K, &wbytes, &nsplit));
+  double2 *work;
+  LQ_TRY(ctx_buf(ctx, "cwork", wbytes, (void **)&work));
+  hipEvent_t ev;
+  LQ_TRY(time_begin(ctx, kCatContract, &ev));
+  LQ_TRY(launch_contract_f64(ctx->stream, (const double2 *)L, ldl, ML, (const double2 *)R, ldr, MR, K, work, nsplit));
+  LQ_TRY(time_end(ctx, kCatContract, ev, 8.0 * ML * (double)MR * (double)K));
+  LQ_TRY(launch_contract_reduce(ctx->stream, work, nsplit, ML, MR, (double2 *)sub, ldsub, 0));
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return collect_timing(ctx);
+}
+
+// ---- 1D moments ----------------------------------------------------------------
+int lq_spectral_moments(lq_csr H, lq_csr OP, const double *phi, int R, int M, int algo, double *mu) {
+  LQ_TRY(check_op(H, "lq_spectral_moments"));
+  if (!phi || R <= 0 || M <= 0 || !mu || (OP && (OP->n != H->n || OP->ctx != H->ctx))) {
+    set_error("lq_spectral_moments: invalid arguments");
+    return LQ_ERR_ARG;
+  }
+  if (algo != LQ_ALGO_DIRECT && algo != LQ_ALGO_DOUBLING) {
+    set_error("lq_spectral_moments: unknown algorithm %d", algo);
+    return LQ_ERR_ARG;
+  }
+  if (algo == LQ_ALGO_DOUBLING && OP) {
+    set_error("lq_spectral_moments: the doubling recursion needs OP = identity");
+    return LQ_ERR_ARG;
+  }
+  lq_ctx ctx = H->ctx;
+  DeviceGuard dg(ctx->device);
+  const int n = H->n;
+  const int g = std::min(R, 64);
+  std::vector<std::complex<double>> acc(M, 0.0);
+  for (int r0 = 0; r0 < R; r0 += g) {
+    const int gg = std::min(g, R - r0);
+    int kp = 1;
+    while (kp < gg) kp <<= 1;
+    const size_t vec = (size_t)n * gg;
+    const int grid = spmm_dot_grid(n, kp);
+    double2 *phi_d, *A, *B, *part0, *part1, *sums0, *sums1;
+    LQ_TRY(ctx_buf(ctx, "phi", vec * sizeof(double2), (void **)&phi_d));
+    LQ_TRY(ctx_buf(ctx, "T0", vec * sizeof(double2), (void **)&A));
+    LQ_TRY(ctx_buf(ctx, "T1", vec * sizeof(double2), (void **)&B));
+    LQ_TRY(upload_states(ctx, phi, n, r0, gg, phi_d));
+    const double2 one = d2(1, 0), zero = d2(0, 0), two = d2(2, 0), mone = d2(-1, 0);
+    if (OP) {
+      VecSet s{phi_d, nullptr, A, gg};
+      LQ_TRY(spmm(ctx, OP, one, zero, &s, 1));
+    } else {
+      LQ_TRY(launch_copy(ctx->stream, A, phi_d, vec));
+    }
+    if (algo == LQ_ALGO_DIRECT) {
+      // slots m = 0..M-1 of width kp
+      LQ_TRY(ctx_buf(ctx, "part0", (size_t)M * grid * kp * sizeof(double2), (void **)&part0));
+      LQ_TRY(ctx_buf(ctx, "sums0", (size_t)M * kp * sizeof(double2), (void **)&sums0));
+      const size_t slot = (size_t)grid * kp;
+      LQ_TRY(launch_dot(ctx->stream, n, gg, phi_d, A, part0, nullptr));
+      if (M > 1) {
+        VecSet s{A, nullptr, B, gg};
+        LQ_TRY(spmm(ctx, H, one, zero, &s, 1, kDotPhi, phi_d, part0 + slot));
+      }
+      for (int m = 1; m + 1 < M; ++m) {
+        // A <- 2 H~ B - A, then swap: after the step B holds T_{m+1}
+        VecSet s{B, A, A, gg};
+        LQ_TRY(spmm(ctx, H, two, mone, &s, 1, kDotPhi, phi_d, part0 + (size_t)(m + 1) * slot));
+        std::swap(A, B);
+      }
+      hipEvent_t ev;
+      LQ_TRY(time_begin(ctx, kCatPost, &ev));
+      LQ_TRY(launch_reduce_partials(ctx->stream, part0, M, grid, kp, sums0));
+      LQ_TRY(time_end(ctx, kCatPost, ev, 0));
+      std::vector<double2> h((size_t)M * kp);
+      LQ_HIP_CHECK(hipMemcpyAsync(h.data(), sums0, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+      LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      for (int m = 0; m < M; ++m) {
+        double scal = 2.0 / R;
+        if (m == 0) scal *= 0.5;
+        for (int v = 0; v < gg; ++v) acc[m] += scal * std::complex<double>(h[(size_t)m * kp + v].x, h[(size_t)m * kp + v].y);
+      }
+    } else {
+      // mu_0 = <T0|T0>, mu_1 = <T0|T1>; step n gives <T_n|T_n>, <T_n|T_n+1>
+      const int nsteps = (M - 1) / 2;
+      LQ_TRY(ctx_buf(ctx, "part0", (size_t)2 * grid * kp * sizeof(double2), (void **)&part0));
+      LQ_TRY(ctx_buf(ctx, "sums0", (size_t)2 * kp * sizeof(double2), (void **)&sums0));
+      LQ_TRY(ctx_buf(ctx, "part1", (size_t)std::max(nsteps, 1) * grid * 2 * kp * sizeof(double2), (void **)&part1));
+      LQ_TRY(ctx_buf(ctx, "sums1", (size_t)std::max(nsteps, 1) * 2 * kp * sizeof(double2), (void **)&sums1));
+      const size_t slot0 = (size_t)grid * kp, slot1 = (size_t)grid * 2 * kp;
+      LQ_TRY(launch_dot(ctx->stream, n, gg, A, A, part0, nullptr));
+      if (M > 1) {
+        VecSet s{A, nullptr, B, gg};
+        LQ_TRY(spmm(ctx, H, one, zero, &s, 1));
+        LQ_TRY(launch_dot(ctx->stream, n, gg, A, B, part0 + slot0, nullptr));
+      }
+      for (int st = 1; st <= nsteps; ++st) {
+        // x = T_st (B), z = y = T_st-1 (A): A <- T_st+1
+        VecSet s{B, A, A, gg};
+        LQ_TRY(spmm(ctx, H, two, mone, &s, 1, kDotDoubling, nullptr, part1 + (size_t)(st - 1) * slot1));
+        std::swap(A, B);
+      }
+      hipEvent_t ev;
+      LQ_TRY(time_begin(ctx, kCatPost, &ev));
+      LQ_TRY(launch_reduce_partials(ctx->stream, part0, 2, grid, kp, sums0));
+      if (nsteps > 0) LQ_TRY(launch_reduce_partials(ctx->stream, part1, nsteps, grid, 2 * kp, sums1));
+      LQ_TRY(time_end(ctx, kCatPost, ev, 0));
+      std::vector<double2> h0((size_t)2 * kp), h1((size_t)std::max(nsteps, 1) * 2 * kp);
+      LQ_HIP_CHECK(hipMemcpyAsync(h0.data(), sums0, sizeof(double2) * h0.size(), hipMemcpyDeviceToHost, ctx->stream));
+      if (nsteps > 0)
+        LQ_HIP_CHECK(hipMemcpyAsync(h1.data(), sums1, sizeof(double2) * (size_t)nsteps * 2 * kp, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+      LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      for (int v = 0; v < gg; ++v) {
+        std::vector<std::complex<double>> m1(M);
+        const std::complex<double> mu0(h0[v].x, h0[v].y), mu1(h0[kp + v].x, h0[kp + v].y);
+        m1[0] = mu0;
+        if (M > 1) m1[1] = mu1;
+        for (int st = 1; st <= nsteps; ++st) {
+          const double2 a = h1[((size_t)(st - 1) * 2 + 0) * kp + v], b = h1[((size_t)(st - 1) * 2 + 1) * kp + v];
+          if (2 * st < M) m1[2 * st] = 2.0 * std::complex<double>(a.x, a.y) - mu0;
+          if (2 * st + 1 < M) m1[2 * st + 1] = 2.0 * std::complex<double>(b.x, b.y) - mu1;
+        }
+        for (int m = 0; m < M; ++m) {
+          double scal = 2.0 / R;
+          if (m == 0) scal *= 0.5;
+          acc[m] += scal * m1[m];
+        }
+      }
+    }
+  }
+  for (int m = 0; m < M; ++m) {
+    mu[2 * m] += acc[m].real();
+    mu[2 * m + 1] += acc[m].imag();
+  }
+  return collect_timing(ctx);
+}
+
+// ---- 2D moments --------------------------------------------------------------------
+static int check_2d_ops(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, const char *what) {
+  LQ_TRY(check_op(H, what));
+  if (nopr <= 0 || !OPR) {
+    set_error("%s: need at least one right operator", what);
+    return LQ_ERR_ARG;
+  }
+  if (OPL && (OPL->n != H->n || OPL->ctx != H->ctx)) {
+    set_error("%s: OPL size/context mismatch", what);
+    return LQ_ERR_ARG;
+  }
+  for (int o = 0; o < nopr; ++o)
+    if (OPR[o] && (OPR[o]->n != H->n || OPR[o]->ctx != H->ctx)) {
+      set_error("%s: OPR[%d] size/context mismatch", what, o);
+      return LQ_ERR_ARG;
+    }
+  return LQ_OK;
+}
+
+int lq_correlation_moments(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, const double *phi, int R, int M0,
+                           int M1, int batch, int precision, double *mu) {
+  LQ_TRY(check_2d_ops(H, OPL, nopr, OPR, "lq_correlation_moments"));
+  if (!phi || R <= 0 || M0 <= 0 || M1 <= 0 || !mu) {
+    set_error("lq_correlation_moments: invalid arguments");
+    return LQ_ERR_ARG;
+  }
+  if (precision != LQ_PREC_FP64) {
+    set_error("lq_correlation_moments: unknown precision %d", precision);
+    return LQ_ERR_ARG;
+  }
+  lq_ctx ctx = H->ctx;
+  DeviceGuard dg(ctx->device);
+  const int B = batch > 0 ? batch : M0;
+  const int Msq = std::max(M0, M1);
+  const int Mt = ord_of(Msq - 1, B) + 1;  // true Chebyshev orders needed
+  const bool padded = (M0 % B != 0) || (M1 % B != 0);
+  const size_t tab = (size_t)Mt * Mt;
+  double2 *raw, *init = nullptr, *outd;
+  LQ_TRY(ctx_buf(ctx, "raw", tab * nopr * sizeof(double2), (void **)&raw));
+  LQ_TRY(ctx_buf(ctx, "mu_out", (size_t)M0 * M1 * sizeof(double2), (void **)&outd));
+  Pipe2D P{ctx, H, OPL, nopr, OPR, H->n, Mt, Mt, raw};
+  LQ_TRY(run_pipe2d(P, phi, R));
+  for (int o = 0; o < nopr; ++o) {
+    if (!padded) {
+      LQ_TRY(ctx_buf(ctx, "mu_init", (size_t)M0 * M1 * sizeof(double2), (void **)&init));
+      LQ_HIP_CHECK(hipMemcpyAsync(init, mu + 2 * (size_t)o * M0 * M1, (size_t)M0 * M1 * sizeof(double2),
+                                  hipMemcpyHostToDevice, ctx->stream));
+    }
+    hipEvent_t ev;
+    LQ_TRY(time_begin(ctx, kCatPost, &ev));
+    LQ_TRY(launch_corr_epilogue(ctx->stream, raw + o * tab, Mt, padded ? nullptr : init, M0, M1, B, R, outd));
+    LQ_TRY(time_end(ctx, kCatPost, ev, 0));
+    LQ_HIP_CHECK(hipMemcpyAsync(mu + 2 * (size_t)o * M0 * M1, outd, (size_t)M0 * M1 * sizeof(double2),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  }
+  return collect_timing(ctx);
+}
+
+// ---- Kubo FFT ------------------------------------------------------------------------
+int lq_kubo_fft(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, const double *phi, int R, int M, int num_sections,
+                int nump, int formula, int precision, double *final_data) {
+  LQ_TRY(check_2d_ops(H, OPL, nopr, OPR, "lq_kubo_fft"));
+  const int n = H->n;
+  if (!phi || R <= 0 || M <= 0 || nump <= 0 || !final_data || num_sections <= 0 || num_sections > n) {
+    set_error("lq_kubo_fft: invalid arguments");
+    return LQ_ERR_ARG;
+  }
+  if (M > nump) {
+    set_error("lq_kubo_fft: M = %d > nump = %d overflows the reference's DFT inputs", M, nump);
+    return LQ_ERR_UNSUPPORTED;
+  }
+  if (formula != LQ_KUBO_GREENWOOD && formula != LQ_KUBO_BASTIN) {
+    set_error("lq_kubo_fft: unknown formula %d", formula);
+    return LQ_ERR_ARG;
+  }
+  if (precision != LQ_PREC_FP64) {
+    set_error("lq_kubo_fft: unknown precision %d", precision);
+    return LQ_ERR_ARG;
+  }
+  if (num_sections > 1 && n % num_sections != 0 && R > 1) {
+    set_error("lq_kubo_fft: with N %% num_sections != 0 and R > 1 the reference reads stale section tails "
+              "(Greenwood_FFTs.cpp:8-11); unsupported");
+    return LQ_ERR_UNSUPPORTED;
+  }
+  lq_ctx ctx = H->ctx;
+  DeviceGuard dg(ctx->device);
+  std::vector<lq_csr> opr(OPR, OPR + nopr), owned;
+  if (num_sections > 1) {
+    for (int o = 0; o < nopr; ++o) {
+      if (!opr[o]) continue;  // identity stays block diagonal
+      lq_csr m = nullptr;
+      LQ_TRY(masked_copy(opr[o], num_sections, &m));
+      owned.push_back(m);
+      opr[o] = m;
+    }
+  }
+  const size_t tab = (size_t)M * M;
+  double2 *raw, *fd;
+  double *coef;
+  int rc = ctx_buf(ctx, "raw", tab * nopr * sizeof(double2), (void **)&raw);
+  if (rc == LQ_OK) rc = ctx_buf(ctx, "kubo_fd", (size_t)2 * nump * nopr * sizeof(double2), (void **)&fd);
+  if (rc == LQ_OK) rc = ctx_buf(ctx, "kubo_coef", (size_t)M * sizeof(double), (void **)&coef);
+  if (rc == LQ_OK) {
+    Pipe2D P{ctx, H, OPL, nopr, opr.data(), n, M, M, raw};
+    rc = run_pipe2d(P, phi, R);
+  }
+  if (rc == LQ_OK) {
+    // c_m = (2 - delta_m0) * Jackson_kernel(m, M) (src/Greenwood_FFTs.cpp:17-18)
+    std::vector<double> c(M);
+    for (int m = 0; m < M; ++m)
+      c[m] = (2 - (m == 0)) *
+             (1 / (M + 1.0) *
+              ((M - m + 1.0) * cos(kPi * m / (M + 1.0)) + sin(kPi * m / (M + 1.0)) / tan(kPi / (M + 1.0))));
+    hipError_t e = hipMemcpyAsync(coef, c.data(), sizeof(double) * M, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+      set_error("lq_kubo_fft: coefficient upload: %s", hipGetErrorString(e));
+      rc = LQ_ERR_HIP;
+    }
+    hipEvent_t ev;
+    if (rc == LQ_OK) rc = time_begin(ctx, kCatPost, &ev);
+    if (rc == LQ_OK) rc = launch_scale(ctx->stream, raw, tab * nopr, 1.0 / R);
+    for (int o = 0; o < nopr && rc == LQ_OK; ++o)
+      rc = launch_kubo_pw(ctx->stream, raw + o * tab, M, M, nump, formula, coef, fd + (size_t)o * 2 * nump);
+    if (rc == LQ_OK) rc = time_end(ctx, kCatPost, ev, 0);
+    if (rc == LQ_OK) {
+      e = hipMemcpyAsync(final_data, fd, (size_t)2 * nump * nopr * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) {
+        set_error("lq_kubo_fft: result download: %s", hipGetErrorString(e));
+        rc = LQ_ERR_HIP;
+      }
+    }
+  }
+  for (auto m : owned) lq_csr_destroy(m);
+  if (rc != LQ_OK) return rc;
+  return collect_timing(ctx);
+}
+
+int lq_kubo_post(lq_ctx ctx, int formula, const double *final_data, int nump, int n, double W, double Ec, double *E,
+                 double *sigma, double *integrand) {
+  if (!ctx || !final_data || nump <= 1 || n <= 0 || !E || !sigma ||
+      (formula != LQ_KUBO_GREENWOOD && formula != LQ_KUBO_BASTIN)) {
+    set_error("lq_kubo_post: invalid arguments");
+    return LQ_ERR_ARG;
+  }
+  DeviceGuard dg(ctx->device);
+  double2 *fd;
+  double *dE, *ds, *di;
+  LQ_TRY(ctx_buf(ctx, "post_fd", (size_t)2 * nump * sizeof(double2), (void **)&fd));
+  LQ_TRY(ctx_buf(ctx, "post_out", (size_t)3 * nump * sizeof(double), (void **)&dE));
+  ds = dE + nump;
+  di = dE + 2 * nump;
+  LQ_HIP_CHECK(hipMemcpyAsync(fd, final_data, (size_t)2 * nump * sizeof(double2), hipMemcpyHostToDevice, ctx->stream));
+  LQ_TRY(launch_kubo_post(ctx->stream, formula, fd, nump, n, W, Ec, dE, ds, di));
+  LQ_HIP_CHECK(hipMemcpyAsync(E, dE, nump * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LQ_HIP_CHECK(hipMemcpyAsync(sigma, ds, nump * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (integrand)
+    LQ_HIP_CHECK(hipMemcpyAsync(integrand, di, nump * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return LQ_OK;
+}
+
+// ---- Jackson + grid post-processors -----------------------------------------------------
+static int jackson_count(int M, double W, double broad) {
+  const double eta = 2.0 * broad / 1000 / W;
+  int maxMom = (int)ceil(kPi / eta);
+  return maxMom > M ? M : maxMom;
+}
+
+int lq_jackson_1d(lq_ctx ctx, double *mu, int M, double W, double broad, int *M_out) {
+  if (!ctx || !mu || M <= 0 || !(broad > 0) || !(W > 0)) {
+    set_error("lq_jackson_1d: invalid arguments");
+    return LQ_ERR_ARG;
+  }
+  DeviceGuard dg(ctx->device);
+  const int mm = jackson_count(M, W, broad);
+  double2 *d;
+  LQ_TRY(ctx_buf(ctx, "jk", (size_t)M * sizeof(double2), (void **)&d));
+  LQ_HIP_CHECK(hipMemcpyAsync(d, mu, (size_t)mm * sizeof(double2), hipMemcpyHostToDevice, ctx->stream));
+  LQ_TRY(launch_jackson_1d(ctx->stream, d, mm));
+  LQ_HIP_CHECK(hipMemcpyAsync(mu, d, (size_t)mm * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (M_out) *M_out = mm;
+  return LQ_OK;
+}
+
+int lq_jackson_2d(lq_ctx ctx, double *mu, int M0, int M1, double W, double b0, double b1, int *M0_out, int *M1_out) {
+  if (!ctx || !mu || M0 <= 0 || M1 <= 0 || !(b0 > 0) || !(b1 > 0) || !(W > 0)) {
+    set_error("lq_jackson_2d: invalid arguments");
+    return LQ_ERR_ARG;
+  }
+  DeviceGuard dg(ctx->device);
+  const int mm0 = jackson_count(M0, W, b0), mm1 = jackson_count(M1, W, b1);
+  double2 *din, *dout;
+  LQ_TRY(ctx_buf(ctx, "jk_in", (size_t)M0 * M1 * sizeof(double2), (void **)&din));
+  LQ_TRY(ctx_buf(ctx, "jk_out", (size_t)mm0 * mm1 * sizeof(double2), (void **)&dout));
+  LQ_HIP_CHECK(hipMemcpyAsync(din, mu, (size_t)M0 * M1 * sizeof(double2), hipMemcpyHostToDevice, ctx->stream));
+  LQ_TRY(launch_jackson_2d(ctx->stream, din, M1, dout, mm0, mm1));
+  LQ_HIP_CHECK(hipMemcpyAsync(mu, dout, (size_t)mm0 * mm1 * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (M0_out) *M0_out = mm0;
+  if (M1_out) *M1_out = mm1;
+  return LQ_OK;
+}
+
+int lq_dos_fftgrid(lq_ctx ctx, const double *mu, int M, int n, double W, double Ec, int num_div, double *E,
+                   double *rho) {
+  if (!ctx || !mu || M <= 0 || num_div <= 0 || !E || !rho) {
+    set_error("lq_dos_fftgrid: invalid arguments");
+    return LQ_ERR_ARG;
+  }
+  DeviceGuard dg(ctx->device);
+  double2 *dm;
+  double *dout;
+  LQ_TRY(ctx_buf(ctx, "grid_mu", (size_t)M * sizeof(double2), (void **)&dm));
+  LQ_TRY(ctx_buf(ctx, "grid_out", (size_t)2 * num_div * sizeof(double), (void **)&dout));
+  LQ_HIP_CHECK(hipMemcpyAsync(dm, mu, (size_t)M * sizeof(double2), hipMemcpyHostToDevice, ctx->stream));
+  LQ_TRY(launch_dos_fftgrid(ctx->stream, dm, M, n, W, Ec, num_div, dout, dout + num_div));
+  LQ_HIP_CHECK(hipMemcpyAsync(E, dout, num_div * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LQ_HIP_CHECK(hipMemcpyAsync(rho, dout + num_div, num_div * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return LQ_OK;
+}
+
+static int grid2d(lq_ctx ctx, int which, const double *mu, int M0, int M1, int n, double W, double Ec, int num_div,
+                  double *E, double *out) {
+  if (!ctx || !mu || M0 <= 0 || M1 <= 0 || num_div <= 1 || !E || !out) {
+    set_error("fftgrid post-processor: invalid arguments");
+    return LQ_ERR_ARG;
+  }
+  DeviceGuard dg(ctx->device);
+  double2 *dm;
+  double *scratch, *dout;
+  LQ_TRY(ctx_buf(ctx, "grid_mu", (size_t)M0 * M1 * sizeof(double2), (void **)&dm));
+  LQ_TRY(ctx_buf(ctx, "grid_scratch", (size_t)2 * num_div * sizeof(double), (void **)&scratch));
+  LQ_TRY(ctx_buf(ctx, "grid_out", (size_t)2 * num_div * sizeof(double), (void **)&dout));
+  LQ_HIP_CHECK(hipMemcpyAsync(dm, mu, (size_t)M0 * M1 * sizeof(double2), hipMemcpyHostToDevice, ctx->stream));
+  if (which == 0)
+    LQ_TRY(launch_greenwood_fftgrid(ctx->stream, dm, M0, M1, n, W, Ec, num_div, scratch, dout, dout + num_div));
+  else
+    LQ_TRY(launch_bastin_fftgrid(ctx->stream, dm, M0, M1, n, W, Ec, num_div, scratch, dout, dout + num_div));
+  LQ_HIP_CHECK(hipMemcpyAsync(E, dout, (num_div - 1) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LQ_HIP_CHECK(hipMemcpyAsync(out, dout + num_div, (num_div - 1) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return LQ_OK;
+}
+
+int lq_greenwood_fftgrid(lq_ctx ctx, const double *mu, int M0, int M1, int n, double W, double Ec, int num_div,
+                         double *E, double *sigma) {
+  return grid2d(ctx, 0, mu, M0, M1, n, W, Ec, num_div, E, sigma);
+}
+
+int lq_bastin_fftgrid(lq_ctx ctx, const double *mu, int M0, int M1, int n, double W, double Ec, int num_div, double *E,
+                      double *sigma) {
+  return grid2d(ctx, 1, mu, M0, M1, n, W, Ec, num_div, E, sigma);
+}
+
+}  // extern "C"

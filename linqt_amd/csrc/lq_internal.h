@@ -1,0 +1,150 @@
+// lq_internal.h -- shared declarations of the gfx950 engine (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/linqt_amd.h"
+
+namespace lq {
+
+void set_error(const char *fmt, ...);
+
+#define LQ_HIP_CHECK(call)                                                                    \
+  do {                                                                                        \
+    hipError_t lq_e_ = (call);                                                                \
+    if (lq_e_ != hipSuccess) {                                                                \
+      ::lq::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #call, hipGetErrorString(lq_e_)); \
+      return LQ_ERR_HIP;                                                                      \
+    }                                                                                         \
+  } while (0)
+
+#define LQ_TRY(expr)              \
+  do {                            \
+    int lq_rc_ = (expr);          \
+    if (lq_rc_ != LQ_OK) return lq_rc_; \
+  } while (0)
+
+// Rows per slice of the sliced-ELL layout.  One slice is one 64-lane
+// wavefront's worth of rows when a lane owns a row.
+constexpr int kSlice = 64;
+
+// Kernel-time categories recorded with HIP events.
+enum TimeCat { kCatSpmv = 0, kCatContract = 1, kCatPost = 2, kNumCats = 3 };
+
+// Device sliced-ELL operator.  Entry j of row r of slice s lives at
+// sptr[s] + j*kSlice + (r % kSlice); padding entries have col = own row and
+// value 0.
+struct SellDev {
+  int n = 0;
+  int nslices = 0;
+  int *sptr = nullptr;   // [nslices+1] entry offsets (multiples of kSlice)
+  int *col = nullptr;    // [stored]
+  double2 *val = nullptr;// [stored]
+  long long stored = 0;
+};
+
+// Arguments of one vector set of a (multi-set) SpMM launch: y = a*A*x + b*z
+// for k interleaved vectors ([n][k] layout).  z may alias y; z == nullptr
+// means b is ignored and y is overwritten.
+struct VecSet {
+  const double2 *x = nullptr;
+  const double2 *z = nullptr;
+  double2 *y = nullptr;
+  int k = 0;
+};
+
+// Dot-product fusion of the recursion kernel.
+enum DotMode {
+  kDotNone = 0,
+  kDotPhi = 1,      // partial = sum_rows conj(phi[i][v]) * y[i][v]
+  kDotDoubling = 2  // partials = sum conj(x_i)x_i and sum conj(x_i) y_i (row's own x)
+};
+
+// Launchers (lq_spmv.hip).  Return LQ_OK or an error code.
+int launch_spmm(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, const VecSet *sets,
+                int nsets, int dot_mode, const double2 *phi, double2 *partial, int *grid_used);
+int spmm_dot_grid(int n, int kpad);  // number of blocks a dot-mode launch uses
+int launch_dot(hipStream_t st, int n, int k, const double2 *a, const double2 *b, double2 *partial,
+               int *grid_used);
+int launch_reduce_partials(hipStream_t st, const double2 *partial, int nslots, int grid, int width,
+                           double2 *out);
+int launch_copy(hipStream_t st, double2 *dst, const double2 *src, size_t count);
+// dst[i*g + v] = src[(r0 + v)*n + i] (state-major device states -> [n][g]).
+int launch_interleave(hipStream_t st, const double2 *src, int n, int r0, int g, double2 *dst);
+
+// Contraction (lq_contract.hip): out[m0*ldo + m1] (+)= sum_kk conj(A[m0][kk]) B[m1][kk].
+int contract_workspace_bytes(int M0, int M1, long long K, size_t *bytes, int *nsplit);
+// Partial tables of the nsplit K-chunks go to work; launch_contract_reduce
+// sums them (deterministically) into out, adding to it when accumulate != 0.
+int launch_contract_f64(hipStream_t st, const double2 *A, long long lda, int M0, const double2 *B,
+                        long long ldb, int M1, long long K, double2 *work, int nsplit);
+int launch_contract_reduce(hipStream_t st, const double2 *work, int nsplit, int M0, int M1, double2 *out,
+                           int ldo, int accumulate);
+
+// Post-processing (lq_post.hip).
+int launch_corr_epilogue(hipStream_t st, const double2 *raw, int ldr, const double2 *init, int M0,
+                         int M1, int B, int R, double2 *out);
+int launch_scale(hipStream_t st, double2 *x, size_t count, double s);
+int launch_kubo_pw(hipStream_t st, const double2 *mu, int ldm, int M, int nump, int formula,
+                   const double *coef, double2 *out);
+int launch_kubo_post(hipStream_t st, int formula, const double2 *final_data, int nump, int n, double W,
+                     double Ec, double *E, double *sigma, double *integrand);
+int launch_jackson_1d(hipStream_t st, double2 *mu, int maxMom);
+int launch_jackson_2d(hipStream_t st, const double2 *mu_in, int M1_in, double2 *mu_out, int mm0, int mm1);
+int launch_dos_fftgrid(hipStream_t st, const double2 *mu, int M, int n, double W, double Ec, int num_div,
+                       double *E, double *rho);
+// scratch: 2*num_div doubles; E, out: num_div-1 doubles each.
+int launch_greenwood_fftgrid(hipStream_t st, const double2 *mu, int M0, int M1, int n, double W, double Ec,
+                             int num_div, double *scratch, double *E, double *out);
+int launch_bastin_fftgrid(hipStream_t st, const double2 *mu, int M0, int M1, int n, double W, double Ec,
+                          int num_div, double *scratch, double *E, double *out);
+
+}  // namespace lq
+
+// Context: one device, one stream, grow-only named scratch buffers.
+struct lq_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool timing = false;
+  lq_timing tsum{};
+  struct Pending {
+    int cat;
+    hipEvent_t a, b;
+    double bytes_or_flops;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  struct Buf {
+    void *p = nullptr;
+    size_t bytes = 0;
+  };
+  std::unordered_map<std::string, Buf> bufs;
+};
+
+// Host-side record of the completed operator (kept for rescale and for
+// section masking) plus its device sliced-ELL image.
+struct lq_csr_s {
+  lq_ctx ctx = nullptr;
+  int n = 0;
+  int fill = 0;
+  // completed entries in row order, columns ascending within a row
+  std::vector<int> rowptr;
+  std::vector<int> col;
+  std::vector<std::complex<double>> val;
+  std::vector<signed char> kind;  // 0: as stored (upper or general), 1: mirrored conj, 2: diagonal
+  lq::SellDev dev;
+};
+
+namespace lq {
+// Context helpers (lq_api.hip).
+int ctx_buf(lq_ctx ctx, const char *name, size_t bytes, void **p);
+int time_begin(lq_ctx ctx, int cat, hipEvent_t *ev);
+int time_end(lq_ctx ctx, int cat, hipEvent_t ev, double amount);
+int collect_timing(lq_ctx ctx);
+}  // namespace lq

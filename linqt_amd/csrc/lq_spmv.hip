@@ -1,0 +1,295 @@
+// lq_spmv.hip -- complex128 sliced-ELL SpMM for the Chebyshev recursion.
+//
+// One launch computes, for up to two vector sets sharing the matrix stream,
+//     y = alpha * A * x + beta * z          (k interleaved vectors per set)
+// which covers SparseMatrixType::Multiply(a, x, b, y) (src/mkl_sparse_matrix.cpp:56-69)
+// and the recursion step T_{n+1} = 2 H~ T_n - T_{n-1} (src/chebyshev_moments.cpp:157-162,
+// src/chebyshev_vectors.cpp:17) for the left and right recursions of
+// CorrelationExpansionMoments at once.
+//
+// Mapping: a 256-thread block owns 256/KP consecutive rows; the KP lanes of a
+// row own one vector each.  A wavefront therefore covers 64/KP rows of one
+// 64-row slice, so the column/value loads of one ELL column j are contiguous
+// across the wave and the x gathers / y stores are 16-byte per lane.
+#include "lq_internal.h"
+
+namespace lq {
+
+namespace {
+
+struct SetArgs {
+  const double2 *x[2];
+  const double2 *z[2];
+  double2 *y[2];
+  int k[2];
+};
+
+__device__ __forceinline__ double2 conj_mul(double2 a, double2 b) {
+  // conj(a) * b
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+template <int KP, int NDOT>
+__device__ __forceinline__ void block_reduce_store(double2 (&d)[2], double2 *partial) {
+  __shared__ double2 sh[2][256];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < NDOT; ++q) sh[q][tid] = d[q];
+  __syncthreads();
+#pragma unroll
+  for (int stride = 128; stride >= KP; stride >>= 1) {
+    if (tid < stride) {
+#pragma unroll
+      for (int q = 0; q < NDOT; ++q) {
+        sh[q][tid].x += sh[q][tid + stride].x;
+        sh[q][tid].y += sh[q][tid + stride].y;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid < KP) {
+#pragma unroll
+    for (int q = 0; q < NDOT; ++q) partial[((size_t)blockIdx.x * NDOT + q) * KP + tid] = sh[q][tid];
+  }
+}
+
+template <int KP, int DOT>
+__global__ __launch_bounds__(256) void k_spmm(int n, const int *__restrict__ sptr, const int *__restrict__ col,
+                                              const double2 *__restrict__ val, double2 alpha, double2 beta,
+                                              SetArgs sa, int nrowgroups, const double2 *__restrict__ phi,
+                                              double2 *__restrict__ partial) {
+  constexpr int RPB = 256 / KP;
+  constexpr int NDOT = (DOT == kDotDoubling) ? 2 : 1;
+  const int tid = threadIdx.x;
+  const int v = tid % KP;
+  const int set = (v < sa.k[0]) ? 0 : 1;
+  const int vv = set ? v - sa.k[0] : v;
+  const int kk = sa.k[set];
+  const bool lane_on = v < sa.k[0] + sa.k[1];
+  const double2 *__restrict__ x = sa.x[set];
+  const double2 *z = sa.z[set];
+  double2 *y = sa.y[set];
+  double2 d[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+
+  for (int g = blockIdx.x; g < nrowgroups; g += gridDim.x) {
+    const int row = g * RPB + tid / KP;
+    if (row < n && lane_on) {
+      const int s = row >> 6, r = row & 63;
+      const int base = sptr[s];
+      const int w = (sptr[s + 1] - base) >> 6;
+      const int *__restrict__ cp = col + base + r;
+      const double2 *__restrict__ vp = val + base + r;
+      double ar = 0.0, ai = 0.0;
+      for (int j = 0; j < w; ++j) {
+        const int c = cp[j * kSlice];
+        const double2 a = vp[j * kSlice];
+        const double2 xv = x[(size_t)c * kk + vv];
+        ar = fma(a.x, xv.x, ar);
+        ar = fma(-a.y, xv.y, ar);
+        ai = fma(a.x, xv.y, ai);
+        ai = fma(a.y, xv.x, ai);
+      }
+      const size_t o = (size_t)row * kk + vv;
+      double2 res = make_double2(alpha.x * ar - alpha.y * ai, alpha.x * ai + alpha.y * ar);
+      if (z) {
+        const double2 zv = z[o];
+        res.x += beta.x * zv.x - beta.y * zv.y;
+        res.y += beta.x * zv.y + beta.y * zv.x;
+      }
+      y[o] = res;
+      if (DOT == kDotPhi) {
+        const double2 t = conj_mul(phi[o], res);
+        d[0].x += t.x;
+        d[0].y += t.y;
+      } else if (DOT == kDotDoubling) {
+        const double2 xi = x[o];
+        const double2 t0 = conj_mul(xi, xi);
+        const double2 t1 = conj_mul(xi, res);
+        d[0].x += t0.x;
+        d[0].y += t0.y;
+        d[1].x += t1.x;
+        d[1].y += t1.y;
+      }
+    }
+  }
+  if (DOT != kDotNone) block_reduce_store<KP, NDOT>(d, partial);
+}
+
+template <int KP>
+__global__ __launch_bounds__(256) void k_dot(int n, int k, const double2 *__restrict__ a,
+                                             const double2 *__restrict__ b, int nrowgroups,
+                                             double2 *__restrict__ partial) {
+  constexpr int RPB = 256 / KP;
+  const int tid = threadIdx.x;
+  const int v = tid % KP;
+  double2 d[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  for (int g = blockIdx.x; g < nrowgroups; g += gridDim.x) {
+    const int row = g * RPB + tid / KP;
+    if (row < n && v < k) {
+      const size_t o = (size_t)row * k + v;
+      const double2 t = conj_mul(a[o], b[o]);
+      d[0].x += t.x;
+      d[0].y += t.y;
+    }
+  }
+  block_reduce_store<KP, 1>(d, partial);
+}
+
+__global__ void k_reduce_partials(const double2 *__restrict__ partial, int nslots, int grid, int width,
+                                  double2 *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nslots * width) return;
+  const int slot = t / width, w = t % width;
+  const double2 *p = partial + (size_t)slot * grid * width + w;
+  double sx = 0.0, sy = 0.0;
+  for (int b = 0; b < grid; ++b) {
+    sx += p[(size_t)b * width].x;
+    sy += p[(size_t)b * width].y;
+  }
+  out[t] = make_double2(sx, sy);
+}
+
+// dst[i*g + v] = src[(r0 + v)*n + i]: state-major -> interleaved block.
+__global__ void k_interleave(const double2 *__restrict__ src, int n, int r0, int g, double2 *__restrict__ dst) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * g) return;
+  const int i = (int)(t / g), v = (int)(t % g);
+  dst[t] = src[(size_t)(r0 + v) * n + i];
+}
+
+int pad_pow2(int k) {
+  int p = 1;
+  while (p < k) p <<= 1;
+  return p;
+}
+
+constexpr int kDotGridCap = 1024;
+
+template <int KP>
+int launch_spmm_kp(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, const SetArgs &sa,
+                   int dot_mode, const double2 *phi, double2 *partial, int *grid_used) {
+  constexpr int RPB = 256 / KP;
+  const int groups = (A.n + RPB - 1) / RPB;
+  int grid = groups;
+  if (dot_mode != kDotNone) grid = groups < kDotGridCap ? groups : kDotGridCap;
+  if (grid_used) *grid_used = grid;
+  switch (dot_mode) {
+    case kDotNone:
+      hipLaunchKernelGGL((k_spmm<KP, kDotNone>), dim3(grid), dim3(256), 0, st, A.n, A.sptr, A.col, A.val, alpha,
+                         beta, sa, groups, phi, partial);
+      break;
+    case kDotPhi:
+      hipLaunchKernelGGL((k_spmm<KP, kDotPhi>), dim3(grid), dim3(256), 0, st, A.n, A.sptr, A.col, A.val, alpha,
+                         beta, sa, groups, phi, partial);
+      break;
+    case kDotDoubling:
+      hipLaunchKernelGGL((k_spmm<KP, kDotDoubling>), dim3(grid), dim3(256), 0, st, A.n, A.sptr, A.col, A.val,
+                         alpha, beta, sa, groups, phi, partial);
+      break;
+    default:
+      set_error("launch_spmm: bad dot mode %d", dot_mode);
+      return LQ_ERR_ARG;
+  }
+  LQ_HIP_CHECK(hipGetLastError());
+  return LQ_OK;
+}
+
+}  // namespace
+
+int spmm_dot_grid(int n, int kpad) {
+  const int RPB = 256 / kpad;
+  const int groups = (n + RPB - 1) / RPB;
+  return groups < kDotGridCap ? groups : kDotGridCap;
+}
+
+int launch_spmm(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, const VecSet *sets, int nsets,
+                int dot_mode, const double2 *phi, double2 *partial, int *grid_used) {
+  if (nsets < 1 || nsets > 2) {
+    set_error("launch_spmm: nsets must be 1 or 2");
+    return LQ_ERR_ARG;
+  }
+  SetArgs sa{};
+  int ktot = 0;
+  for (int s = 0; s < nsets; ++s) {
+    if (!sets[s].x || !sets[s].y || sets[s].k <= 0) {
+      set_error("launch_spmm: incomplete vector set %d", s);
+      return LQ_ERR_ARG;
+    }
+    sa.x[s] = sets[s].x;
+    sa.z[s] = sets[s].z;
+    sa.y[s] = sets[s].y;
+    sa.k[s] = sets[s].k;
+    ktot += sets[s].k;
+  }
+  if (nsets == 1) {
+    sa.x[1] = sa.x[0];
+    sa.z[1] = sa.z[0];
+    sa.y[1] = sa.y[0];
+    sa.k[1] = 0;
+  }
+  if (dot_mode != kDotNone && nsets != 1) {
+    set_error("launch_spmm: dot fusion needs a single vector set");
+    return LQ_ERR_ARG;
+  }
+  const int kp = pad_pow2(ktot);
+  switch (kp) {
+    case 1: return launch_spmm_kp<1>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
+    case 2: return launch_spmm_kp<2>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
+    case 4: return launch_spmm_kp<4>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
+    case 8: return launch_spmm_kp<8>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
+    case 16: return launch_spmm_kp<16>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
+    case 32: return launch_spmm_kp<32>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
+    case 64: return launch_spmm_kp<64>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
+    default:
+      set_error("launch_spmm: %d vectors per row exceed 64", ktot);
+      return LQ_ERR_ARG;
+  }
+}
+
+int launch_dot(hipStream_t st, int n, int k, const double2 *a, const double2 *b, double2 *partial, int *grid_used) {
+  const int kp = pad_pow2(k);
+  const int RPB = 256 / kp;
+  const int groups = (n + RPB - 1) / RPB;
+  const int grid = groups < kDotGridCap ? groups : kDotGridCap;
+  if (grid_used) *grid_used = grid;
+  switch (kp) {
+#define LQ_DOT_CASE(KPV) \
+  case KPV: hipLaunchKernelGGL((k_dot<KPV>), dim3(grid), dim3(256), 0, st, n, k, a, b, groups, partial); break;
+    LQ_DOT_CASE(1)
+    LQ_DOT_CASE(2)
+    LQ_DOT_CASE(4)
+    LQ_DOT_CASE(8)
+    LQ_DOT_CASE(16)
+    LQ_DOT_CASE(32)
+    LQ_DOT_CASE(64)
+#undef LQ_DOT_CASE
+    default:
+      set_error("launch_dot: k = %d exceeds 64", k);
+      return LQ_ERR_ARG;
+  }
+  LQ_HIP_CHECK(hipGetLastError());
+  return LQ_OK;
+}
+
+int launch_reduce_partials(hipStream_t st, const double2 *partial, int nslots, int grid, int width, double2 *out) {
+  const int total = nslots * width;
+  if (total <= 0) return LQ_OK;
+  hipLaunchKernelGGL(k_reduce_partials, dim3((total + 255) / 256), dim3(256), 0, st, partial, nslots, grid, width,
+                     out);
+  LQ_HIP_CHECK(hipGetLastError());
+  return LQ_OK;
+}
+
+int launch_interleave(hipStream_t st, const double2 *src, int n, int r0, int g, double2 *dst) {
+  const size_t total = (size_t)n * g;
+  hipLaunchKernelGGL(k_interleave, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, src, n, r0, g, dst);
+  LQ_HIP_CHECK(hipGetLastError());
+  return LQ_OK;
+}
+
+int launch_copy(hipStream_t st, double2 *dst, const double2 *src, size_t count) {
+  LQ_HIP_CHECK(hipMemcpyAsync(dst, src, count * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  return LQ_OK;
+}
+
+}  // namespace lq

@@ -1,0 +1,252 @@
+"""ctypes binding of the CPU FP64 oracle (``liblq_oracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py``, where it is the checker or the
+timed CPU baseline -- never the thing measured or shipped.
+
+Arrays cross as contiguous numpy complex128 (interleaved re/im doubles).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblq_oracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def build() -> None:
+    """Compiles the oracle in place (``make`` in this directory)."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.isfile(_LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(_LIB_PATH)
+        L.lqo_op_create.restype = C.c_void_p
+        L.lqo_op_create.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp]
+        L.lqo_op_destroy.argtypes = [C.c_void_p]
+        L.lqo_op_nnz.argtypes = [C.c_void_p]
+        L.lqo_op_rescale.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.lqo_op_mv.argtypes = [C.c_void_p, C.c_double, C.c_double, _dp, C.c_double, C.c_double, _dp]
+        L.lqo_op_dense.argtypes = [C.c_void_p, _dp]
+        L.lqo_spectral_moments.argtypes = [C.c_void_p, C.c_void_p, C.c_int, _dp, C.c_int, C.c_int, _dp]
+        L.lqo_correlation_moments.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _dp, C.c_int,
+                                              C.c_int, C.c_int, C.c_int, _dp]
+        L.lqo_dft_backward.argtypes = [C.c_int, _dp, _dp]
+        L.lqo_kubo_jackson.restype = C.c_double
+        L.lqo_kubo_jackson.argtypes = [C.c_int, C.c_int]
+        L.lqo_kubo_fft.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _dp, C.c_int, C.c_int,
+                                   C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.lqo_kubo_post.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.lqo_jackson_1d.argtypes = [_dp, C.c_int, C.c_double, C.c_double]
+        L.lqo_jackson_2d.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _ip, _ip]
+        for fn in ("lqo_dos_fftgrid",):
+            getattr(L, fn).argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
+        for fn in ("lqo_greenwood_fftgrid", "lqo_bastin_fftgrid"):
+            getattr(L, fn).argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
+        L.lqo_omp_threads.restype = C.c_int
+        L.lqo_omp_cheb_step.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.lqo_omp_momtable.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp]
+        L.lqo_omp_spectral_moments.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, C.c_int, _dp]
+        L.lqo_omp_mv.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.lqo_omp_bastin_ffts.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int, _dp]
+        _lib = L
+    return _lib
+
+
+def _d(a: np.ndarray):
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a: np.ndarray):
+    assert a.flags.c_contiguous and a.dtype == np.int32
+    return a.ctypes.data_as(_ip)
+
+
+def _check(rc: int, what: str) -> None:
+    if rc < 0:
+        raise RuntimeError(f"oracle {what} failed with code {rc}")
+
+
+class Op:
+    """An oracle operator: the reference's SparseMatrixType (Hermitian/upper)."""
+
+    def __init__(self, n, rowptr, col, val):
+        self.n = int(n)
+        self._rowptr = np.ascontiguousarray(rowptr, dtype=np.int32)
+        self._col = np.ascontiguousarray(col, dtype=np.int32)
+        self._val = np.ascontiguousarray(val, dtype=np.complex128)
+        self.h = lib().lqo_op_create(self.n, int(self._col.shape[0]), _i(self._rowptr), _i(self._col),
+                                     _d(self._val))
+        if not self.h:
+            raise ValueError("invalid CSR arrays")
+
+    @classmethod
+    def from_csr(cls, m):
+        return cls(m.n, m.rowptr, m.col, m.val)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().lqo_op_destroy(self.h)
+            self.h = None
+
+    def rescale(self, a, b):
+        a, b = complex(a), complex(b)
+        _check(lib().lqo_op_rescale(self.h, a.real, a.imag, b.real, b.imag), "rescale")
+        return self
+
+    def mv(self, a, x, b, y):
+        a, b = complex(a), complex(b)
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.array(y, dtype=np.complex128, copy=True)
+        _check(lib().lqo_op_mv(self.h, a.real, a.imag, _d(x), b.real, b.imag, _d(y)), "mv")
+        return y
+
+    def dense(self):
+        d = np.zeros((self.n, self.n), dtype=np.complex128)
+        _check(lib().lqo_op_dense(self.h, _d(d)), "dense")
+        return d
+
+
+def _h(op):
+    return None if op is None else op.h
+
+
+def spectral_moments(H: Op, OP, phi: np.ndarray, M: int, mu=None) -> np.ndarray:
+    phi = np.ascontiguousarray(np.atleast_2d(phi), dtype=np.complex128)
+    mu = np.zeros(M, dtype=np.complex128) if mu is None else np.array(mu, dtype=np.complex128)
+    _check(lib().lqo_spectral_moments(H.h, _h(OP), H.n, _d(phi), phi.shape[0], M, _d(mu)), "spectral_moments")
+    return mu
+
+
+def correlation_moments(H: Op, OPL, OPR, phi: np.ndarray, M0: int, M1: int, batch: int = 0,
+                        mu=None) -> np.ndarray:
+    phi = np.ascontiguousarray(np.atleast_2d(phi), dtype=np.complex128)
+    mu = np.zeros((M0, M1), dtype=np.complex128) if mu is None else np.array(mu, dtype=np.complex128)
+    _check(lib().lqo_correlation_moments(H.h, _h(OPL), _h(OPR), H.n, _d(phi), phi.shape[0], M0, M1,
+                                         batch, _d(mu)), "correlation_moments")
+    return mu
+
+
+def dft_backward(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.empty_like(x)
+    _check(lib().lqo_dft_backward(x.shape[0], _d(x), _d(out)), "dft")
+    return out
+
+
+def kubo_jackson(m: int, M: int) -> float:
+    return lib().lqo_kubo_jackson(m, M)
+
+
+def kubo_fft(H: Op, OPL, OPR, phi: np.ndarray, M: int, nump: int, formula: int, num_sections: int = 1):
+    """Returns (final_data[2*nump], last_r_data[2*nump])."""
+    phi = np.ascontiguousarray(np.atleast_2d(phi), dtype=np.complex128)
+    fd = np.zeros(2 * nump, dtype=np.complex128)
+    rd = np.zeros(2 * nump, dtype=np.complex128)
+    _check(lib().lqo_kubo_fft(H.h, _h(OPL), _h(OPR), H.n, _d(phi), phi.shape[0], M, num_sections, nump,
+                              formula, _d(fd), _d(rd)), "kubo_fft")
+    return fd, rd
+
+
+def kubo_post(formula: int, final_data: np.ndarray, n: int, W: float, Ec: float):
+    """Returns (E, sigma, integrand) as the reference's output files hold them."""
+    fd = np.ascontiguousarray(final_data, dtype=np.complex128)
+    nump = fd.shape[0] // 2
+    E, s, g = np.zeros(nump), np.zeros(nump), np.zeros(nump)
+    _check(lib().lqo_kubo_post(formula, _d(fd), nump, n, W, Ec, _d(E), _d(s), _d(g)), "kubo_post")
+    return E, s, g
+
+
+def jackson_1d(mu: np.ndarray, W: float, broad: float) -> np.ndarray:
+    mu = np.array(mu, dtype=np.complex128)
+    m = lib().lqo_jackson_1d(_d(mu), mu.shape[0], W, broad)
+    _check(m, "jackson_1d")
+    return mu[:m].copy()
+
+
+def jackson_2d(mu: np.ndarray, W: float, b0: float, b1: float) -> np.ndarray:
+    mu = np.array(mu, dtype=np.complex128)
+    M0, M1 = mu.shape
+    o0, o1 = C.c_int(0), C.c_int(0)
+    _check(lib().lqo_jackson_2d(_d(mu), M0, M1, W, b0, b1, C.byref(o0), C.byref(o1)), "jackson_2d")
+    return mu.reshape(-1)[: o0.value * o1.value].reshape(o0.value, o1.value).copy()
+
+
+def dos_fftgrid(mu: np.ndarray, n: int, W: float, Ec: float, num_div: int):
+    mu = np.ascontiguousarray(mu, dtype=np.complex128)
+    E, rho = np.zeros(num_div), np.zeros(num_div)
+    _check(lib().lqo_dos_fftgrid(_d(mu), mu.shape[0], n, W, Ec, num_div, _d(E), _d(rho)), "dos_fftgrid")
+    return E, rho
+
+
+def greenwood_fftgrid(mu: np.ndarray, n: int, W: float, Ec: float, num_div: int):
+    mu = np.ascontiguousarray(mu, dtype=np.complex128)
+    E, out = np.zeros(num_div - 1), np.zeros(num_div - 1)
+    _check(lib().lqo_greenwood_fftgrid(_d(mu), mu.shape[0], mu.shape[1], n, W, Ec, num_div, _d(E), _d(out)),
+           "greenwood_fftgrid")
+    return E, out
+
+
+def bastin_fftgrid(mu: np.ndarray, n: int, W: float, Ec: float, num_div: int):
+    mu = np.ascontiguousarray(mu, dtype=np.complex128)
+    E, out = np.zeros(num_div - 1), np.zeros(num_div - 1)
+    _check(lib().lqo_bastin_fftgrid(_d(mu), mu.shape[0], mu.shape[1], n, W, Ec, num_div, _d(E), _d(out)),
+           "bastin_fftgrid")
+    return E, out
+
+
+def omp_threads() -> int:
+    return lib().lqo_omp_threads()
+
+
+def omp_cheb_step(H: Op, x: np.ndarray, y: np.ndarray) -> None:
+    """y <- 2 H x - y in place; x, y are (k, n) complex, C-contiguous."""
+    assert x.shape == y.shape and y.flags.c_contiguous and x.flags.c_contiguous
+    _check(lib().lqo_omp_cheb_step(H.h, x.shape[0], _d(x), _d(y)), "omp_cheb_step")
+
+
+def omp_momtable(L: np.ndarray, Rv: np.ndarray, m0_begin: int, m0_end: int) -> np.ndarray:
+    """sub[m0 - m0_begin, m1] = <L[m0]|Rv[m1]> for m0 in [m0_begin, m0_end)."""
+    L = np.ascontiguousarray(L, dtype=np.complex128)
+    Rv = np.ascontiguousarray(Rv, dtype=np.complex128)
+    sub = np.zeros((m0_end - m0_begin, Rv.shape[0]), dtype=np.complex128)
+    _check(lib().lqo_omp_momtable(L.shape[1], Rv.shape[0], _d(L), _d(Rv), m0_begin, m0_end, _d(sub)),
+           "omp_momtable")
+    return sub
+
+
+def omp_mv(op: Op, x: np.ndarray, y: np.ndarray) -> None:
+    """y <- op x in place; x, y are (k, n) complex, C-contiguous."""
+    assert x.shape == y.shape and y.flags.c_contiguous and x.flags.c_contiguous
+    _check(lib().lqo_omp_mv(op.h, x.shape[0], _d(x), _d(y)), "omp_mv")
+
+
+def omp_bastin_ffts(L: np.ndarray, Rv: np.ndarray, nump: int) -> np.ndarray:
+    """Per-site Bastin DFT stage over L, Rv of shape (M, sites); returns r_data (2*nump)."""
+    L = np.ascontiguousarray(L, dtype=np.complex128)
+    Rv = np.ascontiguousarray(Rv, dtype=np.complex128)
+    M, size = L.shape
+    rd = np.zeros(2 * nump, dtype=np.complex128)
+    _check(lib().lqo_omp_bastin_ffts(_d(L), _d(Rv), M, nump, size, _d(rd)), "omp_bastin_ffts")
+    return rd
+
+
+def omp_spectral_moments(H: Op, phi: np.ndarray, M: int) -> np.ndarray:
+    phi = np.ascontiguousarray(np.atleast_2d(phi), dtype=np.complex128)
+    mu = np.zeros(M, dtype=np.complex128)
+    _check(lib().lqo_omp_spectral_moments(H.h, H.n, _d(phi), phi.shape[0], M, _d(mu)), "omp_spectral_moments")
+    return mu

@@ -1,0 +1,125 @@
+/*
+ * lq_oracle.h -- CPU FP64 oracle for the LinQT Chebyshev-moment path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the GPU path is
+ * compared against (tests/, __graft_entry__.smoke(), and the cpu_baseline leg
+ * of bench.py).  No product code links or calls it.
+ *
+ * It restates, in plain C99, the reference's algorithm and loop structure:
+ *   - MKL Hermitian/upper CSR mv ........ src/mkl_sparse_matrix.cpp:56-69, include/sparse_matrix.hpp:50-55
+ *   - Rescale (C = aH + bI) ............. src/mkl_sparse_matrix.cpp:73-97
+ *   - SpectralMoments ................... src/chebyshev_solver.cpp:176-200
+ *   - CorrelationExpansionMoments ....... src/chebyshev_solver.cpp:26-172, src/chebyshev_vectors.cpp:4-51
+ *   - Jackson kernels ................... src/chebyshev_moments1D.cpp:68-87, src/chebyshev_moments2D.cpp:87-117,
+ *                                         include/Kubo_solver_FFT.hpp:99-107
+ *   - Kubo_solver_FFT (Greenwood/Bastin). src/Kubo_solver_FFT.cpp, src/Greenwood_FFTs.cpp, src/Bastin_FFTs.cpp,
+ *                                         src/Kubo_solver_FFT_postProcess.cpp
+ *   - FFT-grid post-processors .......... src/spectralFunctionFromChebmom_FFTgrid.cpp:35-73,
+ *                                         src/kuboGreenwoodFromChebmom_FFTgrid.cpp:35-76,
+ *                                         src/kuboBastinFromChebmom_FFTgrid.cpp:57-106
+ *
+ * Complex numbers cross this interface as interleaved (re, im) doubles.
+ * Every function returns 0 on success and a negative code on bad arguments.
+ */
+#ifndef LQ_ORACLE_H
+#define LQ_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Sparse operator as held by the reference's SparseMatrixType: a copy of the
+ * CSR arrays (zero based, rows sorted).  mv uses MKL's HERMITIAN / UPPER
+ * descriptor semantics: entries with col < row are ignored, every entry with
+ * col > row also acts at (col, row) conjugated. */
+typedef struct lqo_op lqo_op;
+
+lqo_op *lqo_op_create(int n, int nnz, const int *rowptr, const int *col, const double *val);
+void    lqo_op_destroy(lqo_op *op);
+int     lqo_op_nnz(const lqo_op *op);
+/* H <- a*H + b*I (MKL mkl_sparse_z_add of H and b*I); inserts missing diagonals. */
+int     lqo_op_rescale(lqo_op *op, double a_re, double a_im, double b_re, double b_im);
+/* y <- a*Herm(U)*x + b*y.  When b == 0, y is overwritten. */
+int     lqo_op_mv(const lqo_op *op, double a_re, double a_im, const double *x,
+                  double b_re, double b_im, double *y);
+/* Dense Hermitian completion of op (n*n complex, row major) -- tiny n only. */
+int     lqo_op_dense(const lqo_op *op, double *dense);
+
+/* chebyshev::SpectralMoments.  OP == NULL means the identity operator.
+ * phi: R states of length n, state r at phi[2*(r*n + i)].  mu (M complex) is
+ * accumulated into, as the reference's Moments1D is. */
+int lqo_spectral_moments(const lqo_op *H, const lqo_op *OP, int n,
+                         const double *phi, int R, int M, double *mu);
+
+/* chebyshev::CorrelationExpansionMoments (the generator-driven overload).
+ * batch <= 0 selects the default BATCH_SIZE = M0.  mu (M0*M1 complex, row
+ * major m0*M1+m1) is in/out exactly as the reference's Moments2D is:
+ * accumulated into when no padding happens, replaced when it does. */
+int lqo_correlation_moments(const lqo_op *H, const lqo_op *OPL, const lqo_op *OPR,
+                            int n, const double *phi, int R, int M0, int M1,
+                            int batch, double *mu);
+
+/* Unnormalised backward DFT, out[k] = sum_m in[m] exp(+2 pi i m k / len)
+ * (FFTW_BACKWARD definition).  Radix-2 when len is a power of two. */
+int lqo_dft_backward(int len, const double *in, double *out);
+
+/* Kubo_solver_FFT::Jackson_kernel(m, M). */
+double lqo_kubo_jackson(int m, int M);
+
+/* Kubo_solver_FFT::compute, restated with the per-site DFTs.  formula 0 =
+ * KUBO_GREENWOOD, 1 = KUBO_BASTIN.  final_data (2*nump complex) receives the
+ * running mean over states; last_r_data (optional, 2*nump complex) the last
+ * state's r_data. */
+int lqo_kubo_fft(const lqo_op *H, const lqo_op *OPL, const lqo_op *OPR, int n,
+                 const double *phi, int R, int M, int num_sections, int nump,
+                 int formula, double *final_data, double *last_r_data);
+
+/* Kubo_solver_FFT_postProcess: energies written as a*E - Ec (reference
+ * convention), ascending.  sigma receives the file's second column;
+ * integrand (Bastin only, may be NULL) the `_integrand` file column. */
+int lqo_kubo_post(int formula, const double *final_data, int nump, int n,
+                  double W, double Ec, double *E_out, double *sigma_out,
+                  double *integrand_out);
+
+/* Moments1D::ApplyJacksonKernel: returns the truncated moment count
+ * (mu compacted in place). */
+int lqo_jackson_1d(double *mu, int M, double W, double broad_meV);
+/* Moments2D::ApplyJacksonKernel: mu compacted in place to M0o x M1o. */
+int lqo_jackson_2d(double *mu, int M0, int M1, double W, double b0_meV,
+                   double b1_meV, int *M0_out, int *M1_out);
+
+/* FFT-grid post-processors on Jackson-damped moments.  num_div energies
+ * E_i = cos(pi(2i+1/2)/num_div) (DOS: cos(pi(i+1/2)/num_div)).
+ * DOS writes num_div rows; Greenwood and Bastin write num_div-1 rows. */
+int lqo_dos_fftgrid(const double *mu, int M, int n, double W, double Ec,
+                    int num_div, double *E_out, double *rho_out);
+int lqo_greenwood_fftgrid(const double *mu, int M0, int M1, int n, double W,
+                          double Ec, int num_div, double *E_out, double *out);
+int lqo_bastin_fftgrid(const double *mu, int M0, int M1, int n, double W,
+                       double Ec, int num_div, double *E_out, double *out);
+
+/* ---- OpenMP CPU baseline (same algorithm, threads over rows/pairs) ----- */
+int  lqo_omp_threads(void);
+/* One Chebyshev step on a completed operator: y <- 2*H~*x - y for k vectors
+ * (vector r at x[2*(r*n+i)]), row-parallel over the completed CSR. */
+int  lqo_omp_cheb_step(const lqo_op *H, int k, const double *x, double *y);
+/* zdotc table sub(m0,m1) = <L_m0|R_m1> over `count` pairs starting at m0,
+ * i.e. ComputeMomTable restricted to rows [m0_begin, m0_end). */
+int  lqo_omp_momtable(int n, int M1, const double *L, const double *Rv,
+                      int m0_begin, int m0_end, double *sub);
+/* y <- op*x for k vectors (vector r at x[2*(r*n+i)]), row-parallel. */
+int  lqo_omp_mv(const lqo_op *op, int k, const double *x, double *y);
+/* Bastin_FFTs' per-site stage over `size` sites, threads over sites with
+ * per-thread accumulators as the reference does (src/Bastin_FFTs.cpp:120-267).
+ * L, Rv: M vectors of length size (vector m at L[2*(m*size + l)]).
+ * r_data (2*nump complex) is accumulated into. */
+int  lqo_omp_bastin_ffts(const double *L, const double *Rv, int M, int nump, int size,
+                         double *r_data);
+/* Full OpenMP SpectralMoments (row-parallel Hermitian mv + zdotc). */
+int  lqo_omp_spectral_moments(const lqo_op *H, int n, const double *phi,
+                              int R, int M, double *mu);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,219 @@
+"""GPU parity: every C-ABI entry point against the CPU oracle / golden files.
+
+Bar: moments within 1e-10 relative (inf-norm over the table, as the
+reference's moments are O(1) and many vanish by symmetry); the energy-grid
+outputs within 1e-9 relative; the contraction within 1e-12.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, dense_herm, random_upper_csr, rel_inf
+import linqt_amd as lq
+from linqt_amd import io, lattices
+
+pytestmark = pytest.mark.gpu
+
+LC = os.path.join(GOLDEN, "linear_chain")
+TOL_MU = 1e-10
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 8, 16])
+def test_spmv_matches_dense(ctx, k):
+    n = 301
+    rowptr, col, val = random_upper_csr(n, 0.03, seed=20 + k)
+    say("spmv", n, k)
+    A = lq.SparseMatrix(ctx, n, rowptr, col, val)
+    D = dense_herm(n, rowptr, col, val)
+    rng = np.random.default_rng(k)
+    x = rng.normal(size=(n, k)) + 1j * rng.normal(size=(n, k))
+    y = rng.normal(size=(n, k)) + 1j * rng.normal(size=(n, k))
+    for a, b in ((1.0, 0.0), (2.0, -1.0), (0.3 - 0.7j, 1.1 + 0.2j)):
+        got = A.Multiply(a, x, b, y)
+        ref = a * (D @ x) + b * y
+        assert rel_inf(got, ref) < 1e-13, (a, b)
+
+
+def test_spmv_general_fill_and_rescale(ctx, oracle):
+    n = 150
+    rowptr, col, val = random_upper_csr(n, 0.05, seed=31, diag=False)
+    say("rescale", n)
+    A = lq.SparseMatrix(ctx, n, rowptr, col, val)
+    O = oracle.Op(n, rowptr, col, val)
+    A.Rescale(1 / 4.6, -0.3)
+    O.rescale(1 / 4.6, -0.3)
+    x = lattices.random_phase_states(n, 1, seed=2)[0]
+    assert rel_inf(A.Multiply(1.0, x), O.mv(1.0, x, 0.0, np.zeros(n))) < 1e-14
+    # general fill: the arrays are used as stored
+    G = lq.SparseMatrix(ctx, n, rowptr, col, val, fill=lq.LQ_FILL_GENERAL)
+    Dg = np.zeros((n, n), dtype=np.complex128)
+    for i in range(n):
+        for q in range(rowptr[i], rowptr[i + 1]):
+            Dg[i, col[q]] += val[q]
+    assert rel_inf(G.Multiply(1.0, x), Dg @ x) < 1e-14
+
+
+@pytest.mark.parametrize("ML,MR,K", [(70, 50, 1000), (64, 64, 4096 * 9 + 5), (1, 3, 17), (130, 129, 300)])
+def test_compute_mom_table(ctx, ML, MR, K):
+    import ctypes as C
+    from linqt_amd import _lib
+
+    say("mom_table", ML, MR, K)
+    rng = np.random.default_rng(ML + MR)
+    L = rng.normal(size=(ML, K)) + 1j * rng.normal(size=(ML, K))
+    R = rng.normal(size=(MR, K)) + 1j * rng.normal(size=(MR, K))
+    pl, pr, ps = ctx.alloc(L.nbytes), ctx.alloc(R.nbytes), ctx.alloc(ML * MR * 16)
+    try:
+        ctx.upload(pl, L)
+        ctx.upload(pr, R)
+        _lib.call("lq_compute_mom_table", ctx.h, pl, K, ML, pr, K, MR, K, ps, MR)
+        sub = np.zeros((ML, MR), dtype=np.complex128)
+        ctx.download(ps, sub)
+    finally:
+        for p in (pl, pr, ps):
+            ctx.free(p)
+    ref = L.conj() @ R.T
+    assert rel_inf(sub, ref) < 1e-13
+
+
+@pytest.mark.parametrize("M,W,name", [
+    (100, 10.0, "NonLEqOpVX-VXlinear_chainKPM_M100x100.chebmom2D"),
+    (300, 4.9, "NonLEqOpVX-VXlinear_chainKPM_M300x300.chebmom2D"),
+])
+def test_golden_chebmom2d_on_gpu(ctx, M, W, name):
+    say("golden", name)
+    gold, n, _, _ = io.read_chebmom2d(os.path.join(LC, name))
+    H = lq.SparseMatrix.from_file(ctx, os.path.join(LC, "operators", "linear_chain.HAM.CSR"))
+    V = lq.SparseMatrix.from_file(ctx, os.path.join(LC, "operators", "linear_chain.VX.CSR"))
+    H.Rescale(2.0 / W, 0.0)
+    mu = lq.correlation_expansion_moments(H, V, V, io.default_state(n), M, M)
+    assert rel_inf(mu, gold) < 1e-12
+
+
+def _square(ctx, oracle, L, seed):
+    m = lattices.square_anderson(L, W=1.0, seed=seed)
+    A = lq.SparseMatrix.from_csr(ctx, m).Rescale(1 / 4.6, 0.0)
+    O = oracle.Op.from_csr(m).rescale(1 / 4.6, 0.0)
+    return m, A, O
+
+
+@pytest.mark.parametrize("R,M", [(1, 1), (1, 2), (3, 101), (8, 256), (9, 64)])
+def test_spectral_moments_direct_and_doubling(ctx, oracle, R, M):
+    say("spectral", R, M)
+    m, A, O = _square(ctx, oracle, 24, seed=R)
+    phi = lattices.random_phase_states(m.n, R, seed=40 + R)
+    ref = oracle.spectral_moments(O, None, phi, M)
+    got = lq.spectral_moments(A, None, phi, M)
+    assert rel_inf(got, ref) < TOL_MU
+    got2 = lq.spectral_moments(A, None, phi, M, algo=lq.LQ_ALGO_DOUBLING)
+    assert rel_inf(got2, ref) < TOL_MU
+    # accumulation into an existing table, as Moments1D does
+    acc = lq.spectral_moments(A, None, phi, M, mu=np.ones(M))
+    assert rel_inf(acc, ref + 1.0) < TOL_MU
+
+
+def test_spectral_moments_with_operator(ctx, oracle):
+    say("spectral OP")
+    m, A, O = _square(ctx, oracle, 20, seed=3)
+    rowptr, col, val = random_upper_csr(m.n, 0.005, seed=5)
+    OPg = lq.SparseMatrix(ctx, m.n, rowptr, col, val)
+    OPo = oracle.Op(m.n, rowptr, col, val)
+    phi = lattices.random_phase_states(m.n, 4, seed=6)
+    ref = oracle.spectral_moments(O, OPo, phi, 77)
+    assert rel_inf(lq.spectral_moments(A, OPg, phi, 77), ref) < TOL_MU
+    with pytest.raises(lq.LinqtError):
+        lq.spectral_moments(A, OPg, phi, 77, algo=lq.LQ_ALGO_DOUBLING)
+
+
+def _honeycomb(ctx, oracle, L, seed=3):
+    H, Vx, Vy = lattices.haldane_honeycomb(L, seed=seed)
+    G = [lq.SparseMatrix.from_csr(ctx, H).Rescale(1 / 4.6, 0.0), lq.SparseMatrix.from_csr(ctx, Vx),
+         lq.SparseMatrix.from_csr(ctx, Vy)]
+    O = [oracle.Op.from_csr(H).rescale(1 / 4.6, 0.0), oracle.Op.from_csr(Vx), oracle.Op.from_csr(Vy)]
+    return H.n, G, O
+
+
+@pytest.mark.parametrize("M,batch,R", [(24, 0, 1), (24, 0, 3), (20, 6, 2), (20, 7, 2), (33, 16, 1)])
+def test_correlation_moments_vs_oracle(ctx, oracle, M, batch, R):
+    say("corr", M, batch, R)
+    n, G, O = _honeycomb(ctx, oracle, 8)
+    phi = lattices.random_phase_states(n, R, seed=M + R)
+    got = lq.correlation_expansion_moments(G[0], G[1], [G[1], G[2]], phi, M, M, batch=batch)
+    for o, opr in enumerate((O[1], O[2])):
+        ref = oracle.correlation_moments(O[0], O[1], opr, phi, M, M, batch=batch)
+        assert rel_inf(got[o], ref) < TOL_MU, o
+    # the table is Hermitian after the reference's symmetrisation
+    assert np.allclose(got[0], got[0].conj().T, rtol=0, atol=1e-14 * np.abs(got[0]).max())
+
+
+def test_correlation_moments_accumulate_and_identity_ops(ctx, oracle):
+    say("corr accumulate / identity")
+    n, G, O = _honeycomb(ctx, oracle, 6)
+    phi = lattices.random_phase_states(n, 2, seed=5)
+    rng = np.random.default_rng(0)
+    init = rng.normal(size=(16, 16)) + 1j * rng.normal(size=(16, 16))
+    got = lq.correlation_expansion_moments(G[0], None, None, phi, 16, 16, mu=init)
+    ref = oracle.correlation_moments(O[0], None, None, phi, 16, 16, mu=init)
+    assert rel_inf(got, ref) < TOL_MU
+
+
+@pytest.mark.parametrize("formula,M,nump,ns,R", [(0, 16, 16, 1, 2), (1, 16, 16, 1, 2), (1, 12, 64, 1, 3),
+                                                 (0, 20, 40, 2, 2), (1, 8, 32, 3, 1)])
+def test_kubo_fft_vs_oracle(ctx, oracle, formula, M, nump, ns, R):
+    say("kubo", formula, M, nump, ns, R)
+    n, G, O = _honeycomb(ctx, oracle, 6)
+    phi = lattices.random_phase_states(n, R, seed=7)
+    got = lq.kubo_fft(G[0], G[1], [G[1], G[2]], phi, M, nump, formula, num_sections=ns)
+    for o, opr in enumerate((O[1], O[2])):
+        ref, _ = oracle.kubo_fft(O[0], O[1], opr, phi, M, nump, formula, num_sections=ns)
+        assert rel_inf(got[o], ref) < 1e-10, o
+        E, s, g = lq.kubo_post(ctx, formula, got[o], n, 9.2, 0.0)
+        Er, sr, gr = oracle.kubo_post(formula, ref, n, 9.2, 0.0)
+        assert np.allclose(E, Er, rtol=0, atol=1e-13)
+        assert rel_inf(s, sr) < 1e-9
+        if formula == 1:
+            assert rel_inf(g, gr) < 1e-9
+
+
+def test_kubo_rejects_m_above_nump(ctx, oracle):
+    n, G, O = _honeycomb(ctx, oracle, 4)
+    phi = lattices.random_phase_states(n, 1, seed=1)
+    with pytest.raises(lq.LinqtError):
+        lq.kubo_fft(G[0], G[1], G[1], phi, 32, 16, 1)
+
+
+def test_jackson_and_grid_postprocessors(ctx, oracle):
+    say("post-processors")
+    rng = np.random.default_rng(3)
+    M, n, W, Ec = 48, 1234, 7.0, 0.2
+    mu1 = rng.normal(size=M) + 1j * rng.normal(size=M)
+    assert rel_inf(lq.jackson_1d(ctx, mu1, W, 60.0), oracle.jackson_1d(mu1, W, 60.0)) < 1e-14
+    assert len(lq.jackson_1d(ctx, mu1, W, 60.0)) == len(oracle.jackson_1d(mu1, W, 60.0))
+    mu2 = rng.normal(size=(M, M - 4)) + 1j * rng.normal(size=(M, M - 4))
+    j2 = lq.jackson_2d(ctx, mu2, W, 90.0, 40.0)
+    j2o = oracle.jackson_2d(mu2, W, 90.0, 40.0)
+    assert j2.shape == j2o.shape and rel_inf(j2, j2o) < 1e-14
+    for nd in (M, 2 * M + 1):
+        E, r = lq.dos_fftgrid(ctx, mu1, n, W, Ec, nd)
+        Eo, ro = oracle.dos_fftgrid(mu1, n, W, Ec, nd)
+        assert np.allclose(E, Eo, rtol=0, atol=1e-13) and rel_inf(r, ro) < 1e-10
+        E, s = lq.greenwood_fftgrid(ctx, mu2, n, W, Ec, nd)
+        Eo, so = oracle.greenwood_fftgrid(mu2, n, W, Ec, nd)
+        assert np.allclose(E, Eo, rtol=0, atol=1e-13) and rel_inf(s, so) < 1e-9
+        E, s = lq.bastin_fftgrid(ctx, mu2, n, W, Ec, nd)
+        Eo, so = oracle.bastin_fftgrid(mu2, n, W, Ec, nd)
+        assert np.allclose(E, Eo, rtol=0, atol=1e-13) and rel_inf(s, so) < 1e-9
+
+
+def test_bad_arguments_raise(ctx):
+    with pytest.raises(lq.LinqtError):
+        lq.SparseMatrix(ctx, 3, np.array([0, 1, 2, 2]), np.array([0, 5]), np.ones(2))
+    with pytest.raises(lq.LinqtError):
+        lq.SparseMatrix(ctx, 3, np.array([0, 1, 2, 9]), np.array([0, 1]), np.ones(2))

@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--R", type=int, default=None)
     ap.add_argument("--nump", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only time a device-to-device copy of 1 GiB (the reference kernel for PMC byte counts)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -136,6 +138,21 @@ def main():
     ctx = lq.Context(0)
     ctx.set_timing(True)
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
+
+    if args.calibrate:
+        from linqt_amd import _lib
+
+        nbytes = 1 << 30
+        src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+        _lib.call("lq_memcpy_d2d", ctx.h, dst, src, nbytes)
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _lib.call("lq_memcpy_d2d", ctx.h, dst, src, nbytes)
+        dt = (time.perf_counter() - t0) / reps
+        print(json.dumps({"calibrate": "d2d copy", "bytes_read": nbytes, "bytes_written": nbytes, "reps": reps,
+                          "ms": dt * 1e3, "GB/s (read+write)": 2 * nbytes / dt / 1e9}), flush=True)
+        return
 
     if args.workload == "bastin":
         L = args.L or 512

@@ -90,6 +90,7 @@ int lq_malloc(lq_ctx ctx, size_t bytes, void **dptr);
 int lq_free(lq_ctx ctx, void *dptr);
 int lq_memcpy_h2d(lq_ctx ctx, void *dst, const void *src, size_t bytes);
 int lq_memcpy_d2h(lq_ctx ctx, void *dst, const void *src, size_t bytes);
+int lq_memcpy_d2d(lq_ctx ctx, void *dst, const void *src, size_t bytes);
 
 /* ---- sparse operators ---------------------------------------------------- */
 /* Replaces SparseMatrixType::ConvertFromCSR (src/mkl_sparse_matrix.cpp:39-54)

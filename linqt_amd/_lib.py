@@ -58,6 +58,7 @@ _SIGS = {
     "lq_free": (C.c_int, [_vp, _vp]),
     "lq_memcpy_h2d": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     "lq_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "lq_memcpy_d2d": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
     "lq_csr_create": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, _dp, C.c_int, C.POINTER(_vp)]),
     "lq_csr_destroy": (C.c_int, [_vp]),
     "lq_csr_rescale": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double]),

@@ -137,12 +137,14 @@ int upload_sell(lq_csr A) {
   const int n = A->n;
   const int nslices = (n + kSlice - 1) / kSlice;
   std::vector<int> sptr(nslices + 1, 0);
+  int wmax = 0;
   for (int s = 0; s < nslices; ++s) {
     int w = 0;
     for (int r = 0; r < kSlice; ++r) {
       const int i = s * kSlice + r;
       if (i < n) w = std::max(w, A->rowptr[i + 1] - A->rowptr[i]);
     }
+    wmax = std::max(wmax, w);
     const long long next = (long long)sptr[s] + (long long)w * kSlice;
     if (next > 0x7fffffffLL) {
       set_error("operator too large for 32-bit sliced-ELL offsets");
@@ -176,6 +178,7 @@ int upload_sell(lq_csr A) {
   d.n = n;
   d.nslices = nslices;
   d.stored = (long long)stored;
+  d.wmax = wmax;
   LQ_HIP_CHECK(hipMalloc(&d.sptr, sizeof(int) * sptr.size()));
   LQ_HIP_CHECK(hipMalloc(&d.col, sizeof(int) * col.size()));
   LQ_HIP_CHECK(hipMalloc(&d.val, sizeof(double2) * val.size()));
@@ -535,6 +538,14 @@ int lq_memcpy_d2h(lq_ctx ctx, void *dst, const void *src, size_t bytes) {
   if (!ctx || (!dst && bytes) || (!src && bytes)) return LQ_ERR_ARG;
   DeviceGuard g(ctx->device);
   LQ_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return LQ_OK;
+}
+
+int lq_memcpy_d2d(lq_ctx ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return LQ_ERR_ARG;
+  DeviceGuard g(ctx->device);
+  LQ_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
   LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return LQ_OK;
 }

@@ -6,11 +6,17 @@
 // table is one complex GEMM  C = A^H B  on FP64 MFMA (v_mfma_f64_16x16x4_f64):
 //   Re C = Ar Br + Ai Bi,   Im C = Ar Bi - Ai Br.
 //
-// Block tile 64 x 64 (four waves of 32 x 32 = 2 x 2 MFMA tiles).  K is split
-// into nsplit chunks; chunk = blockIdx % nsplit, so with nsplit a multiple of
-// 8 every block of one K-chunk lands on the same XCD and the chunk's rows of
-// A and B are shared in that XCD's L2.  A deterministic second pass sums the
-// chunks.
+// Two kernels: k_contract_f64_lds (128 x 128 block tile staged through LDS,
+// used for tables of at least 128 x 128) and k_contract_f64 (64 x 64 tile,
+// operands straight from global memory, for small tables).  The table is a
+// GEMM with a tiny output and a huge K, so it is bound by how often A and B
+// are re-read: the LDS kernel reads each operand tile once per block and
+// every operand M/128 times in total.  K is split into nsplit chunks;
+// chunk = blockIdx % nsplit keeps each chunk on one XCD.  A deterministic
+// second pass sums the chunks.
+#include <cstdlib>
+#include <cstring>
+
 #include "lq_internal.h"
 
 namespace lq {
@@ -76,10 +82,10 @@ __global__ __launch_bounds__(256) void k_contract_f64(const double2 *__restrict_
       ci[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
     }
 
-  long long kb = k0;
-  for (; kb + 16 <= k1; kb += 16) {
-    const long long kk = kb + ks * 4;
-    double2 a[2][4], b[2][4];
+  // Main loop over pairs of 16-element steps with the next step's operands
+  // loaded while the current step's MFMAs run (two register sets).
+  auto load = [&](double2(&a)[2][4], double2(&b)[2][4], long long kbase) {
+    const long long kk = kbase + ks * 4;
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -87,6 +93,22 @@ __global__ __launch_bounds__(256) void k_contract_f64(const double2 *__restrict_
         a[t][s] = pa[t][kk + s];
         b[t][s] = pb[t][kk + s];
       }
+  };
+  long long kb = k0;
+  const long long kmain = k0 + ((k1 - k0) / 32) * 32;
+  if (kb < kmain) {
+    double2 a0[2][4], b0[2][4], a1[2][4], b1[2][4];
+    load(a0, b0, kb);
+    for (; kb < kmain; kb += 32) {
+      load(a1, b1, kb + 16);
+      mma_step(a0, b0, cr, ci);
+      if (kb + 32 < kmain) load(a0, b0, kb + 32);
+      mma_step(a1, b1, cr, ci);
+    }
+  }
+  for (; kb + 16 <= k1; kb += 16) {
+    double2 a[2][4], b[2][4];
+    load(a, b, kb);
     mma_step(a, b, cr, ci);
   }
   if (kb < k1) {
@@ -103,15 +125,124 @@ __global__ __launch_bounds__(256) void k_contract_f64(const double2 *__restrict_
     mma_step(a, b, cr, ci);
   }
 
-  // D layout of v_mfma_f64_16x16x4: element r of lane l is D[4*(l/16) + r][l % 16].
+  // D layout of v_mfma_f64_16x16x4: element r of lane l is D[l/16 + 4*r][l % 16].
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = m_base + ti * 16 + ks * 4 + r;
+        const int m = m_base + ti * 16 + ks + 4 * r;
         const int nn = n_base + tj * 16 + r16;
+        if (m < M0 && nn < M1) work[((size_t)chunk * M0 + m) * M1 + nn] = make_double2(cr[ti][tj][r], ci[ti][tj][r]);
+      }
+}
+
+// LDS-tiled variant: 512 threads, block tile 128 x 128, waves 2 (m) x 4 (n)
+// of 64 x 32 (4 x 2 MFMA tiles).  K advances in stages of 16 complex
+// elements; a stage of A and B (128 rows each) is loaded once from global
+// memory by the whole block into one of two LDS buffers while the other
+// buffer feeds the MFMAs.  Rows are padded to 17 elements so the 16 rows read
+// by one MFMA fragment land in distinct LDS banks.
+constexpr int kBT = 128;       // block tile (rows of A and of B)
+constexpr int kKC = 16;        // complex elements per stage
+constexpr int kLdsRow = kKC + 1;
+
+__global__ __launch_bounds__(512) void k_contract_f64_lds(const double2 *__restrict__ A, long long lda, int M0,
+                                                          const double2 *__restrict__ B, long long ldb, int M1,
+                                                          long long K, long long kchunk, int nsplit, int tiles_n,
+                                                          double2 *__restrict__ work) {
+  __shared__ double2 sA[2][kBT * kLdsRow];
+  __shared__ double2 sB[2][kBT * kLdsRow];
+  const int bid = blockIdx.x;
+  const int chunk = bid % nsplit;
+  const int tile = bid / nsplit;
+  const int tm = tile / tiles_n, tn = tile % tiles_n;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int r16 = lane & 15, ks = lane >> 4;
+  const int wm = (wave >> 2) * 64, wn = (wave & 3) * 32;
+
+  // global -> LDS assignment: thread t moves row t/4, elements 4*(t%4)..+3
+  const int lrow = tid >> 2, le = (tid & 3) * 4;
+  int ga = tm * kBT + lrow;
+  if (ga >= M0) ga = M0 - 1;  // clamped rows feed outputs that are never stored
+  int gb = tn * kBT + lrow;
+  if (gb >= M1) gb = M1 - 1;
+  const double2 *pa = A + (size_t)ga * lda;
+  const double2 *pb = B + (size_t)gb * ldb;
+
+  const long long k0 = (long long)chunk * kchunk;
+  const long long k1 = (k0 + kchunk < K) ? k0 + kchunk : K;
+  const int nstage = k1 > k0 ? (int)((k1 - k0 + kKC - 1) / kKC) : 0;
+
+  double2 ra[4], rb[4];
+  auto gload = [&](long long kb) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const long long kk = kb + le + s;
+      const bool ok = kk < k1;
+      ra[s] = ok ? pa[kk] : make_double2(0.0, 0.0);
+      rb[s] = ok ? pb[kk] : make_double2(0.0, 0.0);
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      sA[buf][lrow * kLdsRow + le + s] = ra[s];
+      sB[buf][lrow * kLdsRow + le + s] = rb[s];
+    }
+  };
+
+  v4d cr[4][2], ci[4][2];
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) {
+      cr[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
+      ci[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
+    }
+
+  if (nstage > 0) {
+    gload(k0);
+    lstore(0);
+  }
+  __syncthreads();
+  for (int st = 0; st < nstage; ++st) {
+    const int cur = st & 1;
+    const bool more = st + 1 < nstage;
+    if (more) gload(k0 + (long long)(st + 1) * kKC);
+    // lane (r16, ks) feeds k-slot ks of four MFMAs with elements 4*ks + s
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      double2 fa[4], fb[2];
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) fa[ti] = sA[cur][(wm + ti * 16 + r16) * kLdsRow + ks * 4 + s];
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj) fb[tj] = sB[cur][(wn + tj * 16 + r16) * kLdsRow + ks * 4 + s];
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) {
+          cr[ti][tj] = mfma(fa[ti].x, fb[tj].x, cr[ti][tj]);
+          cr[ti][tj] = mfma(fa[ti].y, fb[tj].y, cr[ti][tj]);
+          ci[ti][tj] = mfma(fa[ti].x, fb[tj].y, ci[ti][tj]);
+          ci[ti][tj] = mfma(-fa[ti].y, fb[tj].x, ci[ti][tj]);
+        }
+    }
+    if (more) lstore(cur ^ 1);
+    __syncthreads();
+  }
+
+  // D layout of v_mfma_f64_16x16x4: element r of lane l is D[l/16 + 4*r][l % 16].
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = tm * kBT + wm + ti * 16 + ks + 4 * r;
+        const int nn = tn * kBT + wn + tj * 16 + r16;
         if (m < M0 && nn < M1) work[((size_t)chunk * M0 + m) * M1 + nn] = make_double2(cr[ti][tj][r], ci[ti][tj][r]);
       }
 }
@@ -135,10 +266,22 @@ __global__ void k_contract_reduce(const double2 *__restrict__ work, int nsplit, 
   *o = make_double2(sx, sy);
 }
 
+// Kernel choice: the LDS-tiled 128 x 128 kernel for tables of at least
+// 128 x 128 (LQ_CONTRACT=wave forces the register-only 64 x 64 kernel).
+bool use_lds(int M0, int M1) {
+  static const bool force_wave = [] {
+    const char *e = getenv("LQ_CONTRACT");
+    return e && strcmp(e, "wave") == 0;
+  }();
+  return !force_wave && M0 >= kBT && M1 >= kBT;
+}
+
 int choose_nsplit(int M0, int M1, long long K) {
-  const long long tiles = (long long)((M0 + 63) / 64) * ((M1 + 63) / 64);
-  long long ns = (2048 + tiles - 1) / tiles;  // aim at >= 2048 blocks
-  ns = ((ns + 7) / 8) * 8;                     // whole XCD groups
+  const int bt = use_lds(M0, M1) ? kBT : 64;
+  const int target = use_lds(M0, M1) ? 512 : 2048;  // blocks in flight
+  const long long tiles = (long long)((M0 + bt - 1) / bt) * ((M1 + bt - 1) / bt);
+  long long ns = (target + tiles - 1) / tiles;
+  ns = ((ns + 7) / 8) * 8;  // whole XCD groups
   if (ns > 256) ns = 256;
   long long by_k = K / 4096;  // keep every chunk >= 4096 elements long
   if (by_k < 1) by_k = 1;
@@ -161,16 +304,22 @@ int launch_contract_f64(hipStream_t st, const double2 *A, long long lda, int M0,
     set_error("launch_contract_f64: bad sizes M0=%d M1=%d K=%lld nsplit=%d", M0, M1, K, nsplit);
     return LQ_ERR_ARG;
   }
+  const bool lds = use_lds(M0, M1);
+  const int bt = lds ? kBT : 64;
   long long kchunk = (K + nsplit - 1) / nsplit;
   kchunk = ((kchunk + 15) / 16) * 16;
-  const int tiles_m = (M0 + 63) / 64, tiles_n = (M1 + 63) / 64;
+  const int tiles_m = (M0 + bt - 1) / bt, tiles_n = (M1 + bt - 1) / bt;
   const long long blocks = (long long)tiles_m * tiles_n * nsplit;
   if (blocks > 0x7fffffffLL) {
     set_error("launch_contract_f64: grid too large");
     return LQ_ERR_ARG;
   }
-  hipLaunchKernelGGL(k_contract_f64, dim3((unsigned)blocks), dim3(256), 0, st, A, lda, M0, B, ldb, M1, K, kchunk,
-                     nsplit, tiles_n, work);
+  if (lds)
+    hipLaunchKernelGGL(k_contract_f64_lds, dim3((unsigned)blocks), dim3(512), 0, st, A, lda, M0, B, ldb, M1, K,
+                       kchunk, nsplit, tiles_n, work);
+  else
+    hipLaunchKernelGGL(k_contract_f64, dim3((unsigned)blocks), dim3(256), 0, st, A, lda, M0, B, ldb, M1, K, kchunk,
+                       nsplit, tiles_n, work);
   LQ_HIP_CHECK(hipGetLastError());
   return LQ_OK;
 }

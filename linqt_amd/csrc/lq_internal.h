@@ -47,6 +47,7 @@ struct SellDev {
   int *col = nullptr;    // [stored]
   double2 *val = nullptr;// [stored]
   long long stored = 0;
+  int wmax = 0;          // widest slice (entries per row)
 };
 
 // Arguments of one vector set of a (multi-set) SpMM launch: y = a*A*x + b*z

@@ -11,6 +11,9 @@
 // row own one vector each.  A wavefront therefore covers 64/KP rows of one
 // 64-row slice, so the column/value loads of one ELL column j are contiguous
 // across the wave and the x gathers / y stores are 16-byte per lane.
+#include <cstdlib>
+#include <cstring>
+
 #include "lq_internal.h"
 
 namespace lq {
@@ -53,11 +56,19 @@ __device__ __forceinline__ void block_reduce_store(double2 (&d)[2], double2 *par
   }
 }
 
-template <int KP, int DOT>
+// MI355X has 8 XCDs and dispatches consecutive blocks round-robin over them.
+// Remapping block b to row group (b % 8) * q + b / 8 gives every XCD one
+// contiguous range of rows, so the neighbour gathers of a lattice ordering
+// hit that XCD's own L2.  Logical groups >= ngroups (padding to 8q) exit.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_group(int b, int q) { return (b % kXcds) * q + b / kXcds; }
+
+template <int KP, int DOT, int kUnroll>
 __global__ __launch_bounds__(256) void k_spmm(int n, const int *__restrict__ sptr, const int *__restrict__ col,
                                               const double2 *__restrict__ val, double2 alpha, double2 beta,
                                               SetArgs sa, int nrowgroups, const double2 *__restrict__ phi,
-                                              double2 *__restrict__ partial) {
+                                              double2 *__restrict__ partial, int q) {
   constexpr int RPB = 256 / KP;
   constexpr int NDOT = (DOT == kDotDoubling) ? 2 : 1;
   const int tid = threadIdx.x;
@@ -71,7 +82,10 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int *__restrict__ spt
   double2 *y = sa.y[set];
   double2 d[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
 
-  for (int g = blockIdx.x; g < nrowgroups; g += gridDim.x) {
+  // q > 0: one row group per block, XCD-remapped; q == 0: grid-stride loop
+  const int g0 = q > 0 ? xcd_group(blockIdx.x, q) : blockIdx.x;
+  const int gstep = q > 0 ? nrowgroups : gridDim.x;
+  for (int g = g0; g < nrowgroups; g += gstep) {
     const int row = g * RPB + tid / KP;
     if (row < n && lane_on) {
       const int s = row >> 6, r = row & 63;
@@ -80,14 +94,28 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int *__restrict__ spt
       const int *__restrict__ cp = col + base + r;
       const double2 *__restrict__ vp = val + base + r;
       double ar = 0.0, ai = 0.0;
-      for (int j = 0; j < w; ++j) {
-        const int c = cp[j * kSlice];
-        const double2 a = vp[j * kSlice];
-        const double2 xv = x[(size_t)c * kk + vv];
-        ar = fma(a.x, xv.x, ar);
-        ar = fma(-a.y, xv.y, ar);
-        ai = fma(a.x, xv.y, ai);
-        ai = fma(a.y, xv.x, ai);
+      // Chunks of kUnroll entries: all column/value loads of a chunk are in
+      // flight together, then all of its gathers, so a lattice row costs two
+      // memory latencies instead of one per entry.
+      for (int j0 = 0; j0 < w; j0 += kUnroll) {
+        int c[kUnroll];
+        double2 a[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const bool on = j0 + u < w;
+          c[u] = on ? cp[(j0 + u) * kSlice] : row;
+          a[u] = on ? vp[(j0 + u) * kSlice] : make_double2(0.0, 0.0);
+        }
+        double2 xv[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) xv[u] = (j0 + u < w) ? x[(size_t)c[u] * kk + vv] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          ar = fma(a[u].x, xv[u].x, ar);
+          ar = fma(-a[u].y, xv[u].y, ar);
+          ai = fma(a[u].x, xv[u].y, ai);
+          ai = fma(a[u].y, xv[u].x, ai);
+        }
       }
       const size_t o = (size_t)row * kk + vv;
       double2 res = make_double2(alpha.x * ar - alpha.y * ai, alpha.x * ai + alpha.y * ar);
@@ -113,6 +141,60 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int *__restrict__ spt
     }
   }
   if (DOT != kDotNone) block_reduce_store<KP, NDOT>(d, partial);
+}
+
+// Lane-per-row variant: one lane owns one row and all KS vectors of each of
+// NSETS sets (registers), so each (col, value) pair is loaded once and the
+// x gathers of one row are KS*16 contiguous bytes.
+template <int KS, int NSETS>
+__global__ __launch_bounds__(256) void k_spmm_rows(int n, const int *__restrict__ sptr, const int *__restrict__ col,
+                                                   const double2 *__restrict__ val, double2 alpha, double2 beta,
+                                                   SetArgs sa, int ngroups, int q) {
+  const int g = q > 0 ? xcd_group(blockIdx.x, q) : blockIdx.x;
+  if (g >= ngroups) return;
+  const int row = g * 256 + threadIdx.x;
+  if (row >= n) return;
+  const int s = row >> 6, r = row & 63;
+  const int base = sptr[s];
+  const int w = (sptr[s + 1] - base) >> 6;
+  const int *__restrict__ cp = col + base + r;
+  const double2 *__restrict__ vp = val + base + r;
+  double ar[NSETS][KS], ai[NSETS][KS];
+#pragma unroll
+  for (int st = 0; st < NSETS; ++st)
+#pragma unroll
+    for (int v = 0; v < KS; ++v) ar[st][v] = ai[st][v] = 0.0;
+  for (int j = 0; j < w; ++j) {
+    const int c = cp[j * kSlice];
+    const double2 a = vp[j * kSlice];
+#pragma unroll
+    for (int st = 0; st < NSETS; ++st) {
+      const double2 *__restrict__ xr = sa.x[st] + (size_t)c * KS;
+#pragma unroll
+      for (int v = 0; v < KS; ++v) {
+        const double2 xv = xr[v];
+        ar[st][v] = fma(a.x, xv.x, ar[st][v]);
+        ar[st][v] = fma(-a.y, xv.y, ar[st][v]);
+        ai[st][v] = fma(a.x, xv.y, ai[st][v]);
+        ai[st][v] = fma(a.y, xv.x, ai[st][v]);
+      }
+    }
+  }
+#pragma unroll
+  for (int st = 0; st < NSETS; ++st) {
+    const double2 *z = sa.z[st];
+    double2 *y = sa.y[st] + (size_t)row * KS;
+#pragma unroll
+    for (int v = 0; v < KS; ++v) {
+      double2 res = make_double2(alpha.x * ar[st][v] - alpha.y * ai[st][v], alpha.x * ai[st][v] + alpha.y * ar[st][v]);
+      if (z) {
+        const double2 zv = z[(size_t)row * KS + v];
+        res.x += beta.x * zv.x - beta.y * zv.y;
+        res.y += beta.x * zv.y + beta.y * zv.x;
+      }
+      y[v] = res;
+    }
+  }
 }
 
 template <int KP>
@@ -165,26 +247,52 @@ int pad_pow2(int k) {
 
 constexpr int kDotGridCap = 1024;
 
-template <int KP>
-int launch_spmm_kp(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, const SetArgs &sa,
-                   int dot_mode, const double2 *phi, double2 *partial, int *grid_used) {
+// Schedule knobs, read once: LQ_SPMM_MAPPING=rows selects the lane-per-row
+// mapping, LQ_SPMM_XCD=1 the XCD remap (both measured slower on the
+// lattice operators; kept as schedule options).
+struct SpmmKnobs {
+  bool rows = false;
+  bool xcd = false;
+  SpmmKnobs() {
+    const char *m = getenv("LQ_SPMM_MAPPING");
+    if (m && strcmp(m, "rows") == 0) rows = true;
+    const char *x = getenv("LQ_SPMM_XCD");
+    if (x && strcmp(x, "1") == 0) xcd = true;
+  }
+};
+
+const SpmmKnobs &knobs() {
+  static SpmmKnobs k;
+  return k;
+}
+
+int xcd_q(int groups) { return knobs().xcd ? (groups + kXcds - 1) / kXcds : 0; }
+
+template <int KP, int U>
+int launch_spmm_kpu(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, const SetArgs &sa,
+                    int dot_mode, const double2 *phi, double2 *partial, int *grid_used) {
   constexpr int RPB = 256 / KP;
   const int groups = (A.n + RPB - 1) / RPB;
-  int grid = groups;
-  if (dot_mode != kDotNone) grid = groups < kDotGridCap ? groups : kDotGridCap;
+  int grid = groups, q = 0;
+  if (dot_mode != kDotNone) {
+    grid = groups < kDotGridCap ? groups : kDotGridCap;
+  } else {
+    q = xcd_q(groups);
+    if (q > 0) grid = q * kXcds;
+  }
   if (grid_used) *grid_used = grid;
   switch (dot_mode) {
     case kDotNone:
-      hipLaunchKernelGGL((k_spmm<KP, kDotNone>), dim3(grid), dim3(256), 0, st, A.n, A.sptr, A.col, A.val, alpha,
-                         beta, sa, groups, phi, partial);
+      hipLaunchKernelGGL((k_spmm<KP, kDotNone, U>), dim3(grid), dim3(256), 0, st, A.n, A.sptr, A.col, A.val,
+                         alpha, beta, sa, groups, phi, partial, q);
       break;
     case kDotPhi:
-      hipLaunchKernelGGL((k_spmm<KP, kDotPhi>), dim3(grid), dim3(256), 0, st, A.n, A.sptr, A.col, A.val, alpha,
-                         beta, sa, groups, phi, partial);
+      hipLaunchKernelGGL((k_spmm<KP, kDotPhi, U>), dim3(grid), dim3(256), 0, st, A.n, A.sptr, A.col, A.val,
+                         alpha, beta, sa, groups, phi, partial, 0);
       break;
     case kDotDoubling:
-      hipLaunchKernelGGL((k_spmm<KP, kDotDoubling>), dim3(grid), dim3(256), 0, st, A.n, A.sptr, A.col, A.val,
-                         alpha, beta, sa, groups, phi, partial);
+      hipLaunchKernelGGL((k_spmm<KP, kDotDoubling, U>), dim3(grid), dim3(256), 0, st, A.n, A.sptr, A.col, A.val,
+                         alpha, beta, sa, groups, phi, partial, 0);
       break;
     default:
       set_error("launch_spmm: bad dot mode %d", dot_mode);
@@ -192,6 +300,17 @@ int launch_spmm_kp(hipStream_t st, const SellDev &A, double2 alpha, double2 beta
   }
   LQ_HIP_CHECK(hipGetLastError());
   return LQ_OK;
+}
+
+// Unroll width: the smallest of 4/8/12/16 covering the widest slice (wider
+// operators loop over chunks of 16).
+template <int KP>
+int launch_spmm_kp(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, const SetArgs &sa,
+                   int dot_mode, const double2 *phi, double2 *partial, int *grid_used) {
+  if (A.wmax <= 4) return launch_spmm_kpu<KP, 4>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
+  if (A.wmax <= 8) return launch_spmm_kpu<KP, 8>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
+  if (A.wmax <= 12) return launch_spmm_kpu<KP, 12>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
+  return launch_spmm_kpu<KP, 16>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
 }
 
 }  // namespace
@@ -230,6 +349,38 @@ int launch_spmm(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, c
   if (dot_mode != kDotNone && nsets != 1) {
     set_error("launch_spmm: dot fusion needs a single vector set");
     return LQ_ERR_ARG;
+  }
+  // lane-per-row schedule: equal power-of-two widths per set, no dot fusion
+  const int ks = sets[0].k;
+  const bool rows_ok = knobs().rows && dot_mode == kDotNone && (nsets == 1 || sets[1].k == ks) &&
+                       (ks == 1 || ks == 2 || ks == 4 || ks == 8 || (ks == 16 && nsets == 1));
+  if (rows_ok) {
+    const int groups = (A.n + 255) / 256;
+    const int q = xcd_q(groups);
+    const int grid = q > 0 ? q * kXcds : groups;
+    if (grid_used) *grid_used = grid;
+#define LQ_ROWS(KSV, NS)                                                                                         \
+  hipLaunchKernelGGL((k_spmm_rows<KSV, NS>), dim3(grid), dim3(256), 0, st, A.n, A.sptr, A.col, A.val, alpha, beta, \
+                     sa, groups, q)
+    if (nsets == 1) {
+      switch (ks) {
+        case 1: LQ_ROWS(1, 1); break;
+        case 2: LQ_ROWS(2, 1); break;
+        case 4: LQ_ROWS(4, 1); break;
+        case 8: LQ_ROWS(8, 1); break;
+        default: LQ_ROWS(16, 1); break;
+      }
+    } else {
+      switch (ks) {
+        case 1: LQ_ROWS(1, 2); break;
+        case 2: LQ_ROWS(2, 2); break;
+        case 4: LQ_ROWS(4, 2); break;
+        default: LQ_ROWS(8, 2); break;
+      }
+    }
+#undef LQ_ROWS
+    LQ_HIP_CHECK(hipGetLastError());
+    return LQ_OK;
   }
   const int kp = pad_pow2(ktot);
   switch (kp) {

@@ -223,16 +223,26 @@ def main():
                      "TFLOP/s": ctr_tfs, "frac_fp64_mfma": ctr_tfs / FP64_MFMA_PEAK_TFS},
         "post_ms_per_step": tm["post_ms"] / args.steps,
     }
+    # HBM bytes per launch of the dominant kernel, from the committed PMC summary of this workload
+    traffic_tab = {}
+    try:
+        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")) as f:
+            t = json.load(f)
+        if t.get("workload") == config["workload"]:
+            traffic_tab = t["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        traffic_tab = {}
     if tm["contract_ms"] > tm["spmv_ms"]:
         per_launch = tm["contract_flops"] / max(tm["contract_launches"], 1)
-        roofline = {"bound": "mfma", "kernel": "k_contract_f64", "achieved": ctr_tfs, "peak": FP64_MFMA_PEAK_TFS,
-                    "unit": "TFLOP/s", "frac": ctr_tfs / FP64_MFMA_PEAK_TFS, "traffic": None,
+        kname = "k_contract_f64_lds" if min(config["M"], config["M"]) >= 128 else "k_contract_f64"
+        roofline = {"bound": "mfma", "kernel": kname, "achieved": ctr_tfs, "peak": FP64_MFMA_PEAK_TFS,
+                    "unit": "TFLOP/s", "frac": ctr_tfs / FP64_MFMA_PEAK_TFS, "traffic": traffic_tab.get(kname),
                     "per_launch": f"{per_launch:.4g} flop (8*M^2*N*R)",
                     "avg_launch_ms": tm["contract_ms"] / max(tm["contract_launches"], 1)}
     else:
         per_launch = tm["spmv_bytes"] / max(tm["spmv_launches"], 1)
         roofline = {"bound": "hbm", "kernel": "k_spmm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None,
+                    "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": traffic_tab.get("k_spmm<8, 0, 12>"),
                     "per_launch": f"{per_launch:.4g} B (20 nnz + 4(N+1) + 48 k N)",
                     "avg_launch_ms": tm["spmv_ms"] / max(tm["spmv_launches"], 1)}
 

@@ -1,0 +1,3 @@
+// chebyshev_solver.hpp -- same name as the reference header; the declarations live in linqt.hpp.
+#pragma once
+#include "linqt.hpp"

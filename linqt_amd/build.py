@@ -42,6 +42,42 @@ def _compile(src: str) -> str:
     return obj
 
 
+CXX_LIB = os.path.join(HERE, "liblinqt_cxx.so")
+BIN = os.path.join(HERE, "bin")
+ROOT = os.path.dirname(HERE)
+CXXFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I", os.path.join(ROOT, "include", "linqt"),
+            "-I", os.path.join(ROOT, "include")]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{' '.join(cmd)} failed:\n{r.stdout}\n{r.stderr}")
+
+
+def build_cxx(verbose: bool = False) -> None:
+    """The C++ facade (reference class names) and the reference's CLI drivers,
+    host-only code over the C ABI."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = CXX_LIB + ".tmp"
+    _run([cxx, *CXXFLAGS, "-shared", os.path.join(HERE, "cpp", "linqt_facade.cpp"), "-o", tmp,
+          "-L", HERE, "-llinqt_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, CXX_LIB)
+    os.makedirs(BIN, exist_ok=True)
+    drivers = sorted(glob.glob(os.path.join(HERE, "cpp", "drivers", "*.cpp")))
+
+    def one(src):
+        exe = os.path.join(BIN, os.path.basename(src)[:-4])
+        _run([cxx, *CXXFLAGS, src, "-o", exe + ".tmp", "-L", HERE, "-llinqt_cxx", "-llinqt_amd",
+              "-Wl,-rpath,$ORIGIN/.."])
+        os.replace(exe + ".tmp", exe)
+
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(drivers))) as ex:
+        list(ex.map(one, drivers))
+    if verbose:
+        print("built", CXX_LIB, "and", len(drivers), "drivers in", BIN)
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
@@ -55,6 +91,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_cxx(verbose)
     return LIB
 
 

@@ -168,13 +168,16 @@ __global__ void k_kubo_post(int formula, const double2 *__restrict__ fd, int num
       if (integ) integ[j] = 0.0;
     }
   }
+  if (formula != LQ_KUBO_BASTIN) return;
+  // ascending energies in LDS, then the left-Riemann cumulative sum in the
+  // reference's order on one thread
+  extern __shared__ double er[];
+  for (int j = threadIdx.x; j < nump; j += blockDim.x) er[j] = cos(kPi * (2 * crescent_src(j, nump) + 0.5) / nump);
   __syncthreads();
-  if (formula == LQ_KUBO_BASTIN && threadIdx.x == 0) {
+  if (threadIdx.x == 0) {
     double acc = 0;
     for (int i = 0; i < nump - 1; ++i) {
-      const int k0 = crescent_src(i, nump), k1 = crescent_src(i + 1, nump);
-      const double e0 = cos(kPi * (2 * k0 + 0.5) / nump), e1 = cos(kPi * (2 * k1 + 0.5) / nump);
-      acc += integ[i] * (e1 - e0);
+      acc += integ[i] * (er[i + 1] - er[i]);
       so[i] = acc;
     }
     so[nump - 1] = 0.0;
@@ -345,7 +348,12 @@ int launch_kubo_pw(hipStream_t st, const double2 *mu, int ldm, int M, int nump, 
 
 int launch_kubo_post(hipStream_t st, int formula, const double2 *final_data, int nump, int n, double W, double Ec,
                      double *E, double *sigma, double *integrand) {
-  hipLaunchKernelGGL(k_kubo_post, dim3(1), dim3(256), 0, st, formula, final_data, nump, n, W, Ec, E, sigma, integrand);
+  if ((size_t)nump * sizeof(double) > 64 * 1024) {
+    set_error("kubo_post: nump = %d exceeds 8192", nump);
+    return LQ_ERR_ARG;
+  }
+  hipLaunchKernelGGL(k_kubo_post, dim3(1), dim3(256), (size_t)nump * sizeof(double), st, formula, final_data, nump, n,
+                     W, Ec, E, sigma, integrand);
   LQ_HIP_CHECK(hipGetLastError());
   return LQ_OK;
 }

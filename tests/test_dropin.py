@@ -1,0 +1,123 @@
+"""Drop-in tests: the reference's CLI contract (operators/ + BOUNDS + state
+files in the working directory, same output names and formats), run through
+the C++ facade and the C ABI, checked against the golden files / the oracle."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, rel_inf
+from linqt_amd import io, lattices
+
+BIN = os.path.join(ROOT, "linqt_amd", "bin")
+LC = os.path.join(GOLDEN, "linear_chain")
+
+
+def _run(exe, args, cwd, env=None):
+    e = dict(os.environ)
+    e.update(env or {})
+    print("[case]", exe, *args, flush=True)
+    r = subprocess.run([os.path.join(BIN, exe), *map(str, args)], cwd=cwd, capture_output=True, text=True,
+                       timeout=300, env=e)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r
+
+
+def _two_col(path):
+    a = np.loadtxt(path)
+    return a[:, 0], a[:, 1]
+
+
+def test_drivers_print_usage_without_gpu():
+    """The binaries load and link (no GPU call happens on the usage path)."""
+    for exe in ("inline_compute-kpm-spectralOp", "inline_compute-kpm-nonEqOp", "inline_compute-Kubo-kpm-FFT",
+                "spectralFunctionFromChebmom_FFTgrid", "kuboGreenwoodFromChebmom_FFTgrid",
+                "kuboBastinFromChebmom_FFTgrid"):
+        r = subprocess.run([os.path.join(BIN, exe)], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0 and "should be called" in r.stdout, exe
+
+
+@pytest.fixture()
+def chain_dir(tmp_path):
+    os.makedirs(tmp_path / "operators")
+    for f in ("linear_chain.HAM.CSR", "linear_chain.VX.CSR"):
+        shutil.copy(os.path.join(LC, "operators", f), tmp_path / "operators" / f)
+    return tmp_path
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("M,bounds,golden", [
+    (100, "-5 5", "NonLEqOpVX-VXlinear_chainKPM_M100x100.chebmom2D"),
+    (300, "-2.45 2.45", "NonLEqOpVX-VXlinear_chainKPM_M300x300.chebmom2D"),
+])
+def test_nonEqOp_driver_reproduces_golden(chain_dir, M, bounds, golden):
+    (chain_dir / "BOUNDS").write_text(bounds + "\n")
+    _run("inline_compute-kpm-nonEqOp", ["linear_chain", "VX", "VX", M], chain_dir)
+    out = chain_dir / f"NonEqOpVX-VXlinear_chainKPM_M{M}x{M}_statedefault.chebmom2D"
+    mu, n, W, Ec = io.read_chebmom2d(str(out))
+    gold, gn, gW, gEc = io.read_chebmom2d(os.path.join(LC, golden))
+    assert (n, W, Ec) == (gn, gW, gEc)
+    assert rel_inf(mu, gold) < 1e-12
+    # the FFT-grid post-processors on the produced file
+    _run("kuboGreenwoodFromChebmom_FFTgrid", [out.name, 50], chain_dir)
+    _run("kuboBastinFromChebmom_FFTgrid", [out.name, 50], chain_dir)
+
+
+@pytest.mark.gpu
+def test_post_processor_drivers_match_oracle(chain_dir, oracle):
+    g = os.path.join(LC, "NonLEqOpVX-VXlinear_chainKPM_M100x100.chebmom2D")
+    shutil.copy(g, chain_dir / "g.chebmom2D")
+    _run("kuboGreenwoodFromChebmom_FFTgrid", ["g.chebmom2D", 80], chain_dir)
+    _run("kuboBastinFromChebmom_FFTgrid", ["g.chebmom2D", 80], chain_dir)
+    mu, n, W, Ec = io.read_chebmom2d(g)
+    muJ = oracle.jackson_2d(mu, W, 80.0, 80.0)
+    nd = max(muJ.shape)
+    for name, fn in (("KuboGreenWood_gJACKSON_FFTgrid.dat", oracle.greenwood_fftgrid),
+                     ("KuboBastin_gJACKSON_FFTgrid.dat", oracle.bastin_fftgrid)):
+        E, s = _two_col(chain_dir / name)
+        Eo, so = fn(muJ, n, W, Ec, nd)
+        assert np.allclose(E, Eo, rtol=1e-5, atol=1e-9)
+        assert rel_inf(s, so) < 1e-5  # files hold 6 significant digits
+
+
+@pytest.mark.gpu
+def test_spectralOp_driver_with_state_file(tmp_path, oracle):
+    m = lattices.square_anderson(16, W=1.0, seed=11)
+    os.makedirs(tmp_path / "operators")
+    io.write_csr(str(tmp_path / "operators" / "sq.HAM.CSR"), m)
+    (tmp_path / "BOUNDS").write_text("-4.6 4.6\n")
+    phi = lattices.random_phase_states(m.n, 3, seed=12)
+    io.write_vector_state_file(str(tmp_path / "phases"), phi)
+    _run("inline_compute-kpm-spectralOp", ["sq", "1", 64, "phases"], tmp_path)
+    mu, n, W, Ec = io.read_chebmom1d(str(tmp_path / "SpectralOp1sqKPM_M64_statephases.chebmom1D"))
+    H = io.read_csr(str(tmp_path / "operators" / "sq.HAM.CSR"))
+    Ho = oracle.Op.from_csr(H).rescale(1 / 4.6, 0.0)
+    phi_file = io.read_state_file(str(tmp_path / "phases"), m.n)
+    ref = oracle.spectral_moments(Ho, None, phi_file, 64)
+    assert (n, W, Ec) == (m.n, 9.2, 0.0) and rel_inf(mu, ref) < 1e-12
+    _run("spectralFunctionFromChebmom_FFTgrid", ["SpectralOp1sqKPM_M64_statephases.chebmom1D", 200], tmp_path)
+    E, rho = _two_col(tmp_path / "meanSpectralOp1sqKPM_M64_statephasesJACKSON_FFTgrid.dat")
+    muJ = oracle.jackson_1d(mu, W, 200.0)
+    Eo, ro = oracle.dos_fftgrid(muJ, n, W, Ec, len(muJ))
+    assert np.allclose(E, Eo, rtol=1e-5, atol=1e-9) and rel_inf(rho, ro) < 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("formula", ["greenwood", "bastin"])
+def test_kubo_fft_driver_matches_oracle(chain_dir, oracle, formula):
+    (chain_dir / "BOUNDS").write_text("-5 5\n")
+    M = 48
+    env = {"LQ_KUBO_FORMULA": formula} if formula == "bastin" else {}
+    _run("inline_compute-Kubo-kpm-FFT", ["linear_chain", "VX", "VX", M], chain_dir, env)
+    prefix = "Bastin_FFT" if formula == "bastin" else "Greenwood_FFT"
+    E, s = _two_col(chain_dir / f"{prefix}VX-VXlinear_chainKPM_M{M}x{M}_statedefault.conductivity")
+    H = io.read_csr(os.path.join(LC, "operators", "linear_chain.HAM.CSR"))
+    V = io.read_csr(os.path.join(LC, "operators", "linear_chain.VX.CSR"))
+    Ho, Vo = oracle.Op.from_csr(H).rescale(0.2, 0.0), oracle.Op.from_csr(V)
+    fid = 1 if formula == "bastin" else 0
+    fd, _ = oracle.kubo_fft(Ho, Vo, Vo, io.default_state(H.n), M, M, fid)
+    Eo, so, _ = oracle.kubo_post(fid, fd, H.n, 10.0, 0.0)
+    assert np.allclose(E, Eo, rtol=1e-5, atol=1e-9)
+    assert rel_inf(s, so) < 1e-5

@@ -13,6 +13,12 @@ formula on 2048 energies.  One step = one full solve of both components
 `--workload dos` runs the DOS configuration instead (square 256 x 256,
 M = 1024, R = 8, Jackson + DOS on 2048 energies; value = M * R / s).
 
+`--workload batch` runs 64 independent DOS instances (cubic Anderson
+L in {16..96}, Kane-Mele-like honeycomb N = 16k..1M, M in {256..2048},
+R in {1..16}, drawn from a fixed seed) through the batched engine; one step
+is the whole batch, value = sum of M * R over the jobs / s.  The same jobs
+run one after another through the single-job solver once, for comparison.
+
 Prints one JSON line.  The roofline object describes the kernel class that
 takes most of the step time, measured with HIP events on the library's own
 stream; cpu_baseline times the CPU restatement of the reference algorithm on
@@ -30,6 +36,8 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
+INT8_MFMA_PEAK_TOPS = 5000.0
+PRECISIONS = {"fp64": 0, "int8x4": 4, "int8x5": 5, "int8x6": 6, "int8x7": 7}
 
 
 def _pct(xs, q):
@@ -91,6 +99,63 @@ def cpu_baseline_bastin(host_ops, M, R, nump, phi):
     }
 
 
+def batch_instances(lq, ctx, njobs):
+    """The 64-instance mix: even jobs cubic Anderson, odd jobs Kane-Mele-like."""
+    from linqt_amd import lattices
+
+    rng = np.random.default_rng(20261205)
+    specs, mats = [], []
+    for i in range(njobs):
+        if i % 2 == 0:
+            L = int(rng.choice([16, 24, 32, 48, 64, 96]))
+            W = float(rng.uniform(1.0, 4.0))
+            Hh = lattices.cubic_anderson(L, W=W, seed=20261205 + i)
+            a = 6.0 + W / 2 + 0.01
+            kind = "cubic"
+        else:
+            L = int(rng.choice([64, 91, 128, 181, 256, 362, 512]))
+            W = float(rng.uniform(0.5, 2.0))
+            Hh = lattices.kane_mele(L, W=W, seed=20261205 + i)
+            a = 4.6
+            kind = "kanemele"
+        M = int(rng.choice([256, 512, 1024, 2048]))
+        R = int(rng.choice([1, 2, 4, 8, 16]))
+        H = lq.SparseMatrix.from_csr(ctx, Hh).Rescale(1 / a, 0.0)
+        phi = lattices.random_phase_states(Hh.n, R, seed=20261205 + 1000 + i)
+        specs.append({"kind": kind, "L": L, "W": W, "a": a, "M": M, "R": R})
+        mats.append((H, Hh, lq.DeviceStates(ctx, phi)))
+    return specs, mats
+
+
+def cpu_baseline_batch(specs, mats):
+    """Times the CPU restatement on one state of a mid-size instance for 64
+    steps and extrapolates by the row-step work (nnz + N R) of the batch."""
+    import oracle as O
+
+    k = min(range(len(specs)), key=lambda i: abs(mats[i][1].n - 100000))
+    Hh = mats[k][1]
+    Ho = O.Op.from_csr(Hh).rescale(1 / specs[k]["a"], 0.0)
+    phi = np.zeros((1, Hh.n), complex)
+    phi[0, 0] = 1.0
+    O.omp_spectral_moments(Ho, phi, 2)
+    steps = 64
+    t0 = time.perf_counter()
+    O.omp_spectral_moments(Ho, phi, steps)
+    t = time.perf_counter() - t0
+    unit = t / (steps * (2 * Hh.nnz - Hh.n + Hh.n))
+    work = sum(sp["M"] * (sp["R"] * (2 * m[1].nnz - m[1].n) + m[1].n * sp["R"]) for sp, m in zip(specs, mats))
+    step = unit * work
+    return {
+        "value": sum(sp["M"] * sp["R"] for sp in specs) / step,
+        "unit": "moments/s",
+        "cores": O.omp_threads(),
+        "kind": "port",
+        "sample": (f"CPU restatement of SpectralMoments (OpenMP, {O.omp_threads()} threads): 1 state x {steps} "
+                   f"moments of {specs[k]['kind']}{specs[k]['L']} (N={Hh.n}) took {t:.3f} s; extrapolated by "
+                   f"row-step work (nnz + N per state-step) to the whole batch = {step:.1f} s"),
+    }
+
+
 def cpu_baseline_dos(host_H, M, R, phi):
     import oracle as O
 
@@ -116,11 +181,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("bastin", "dos"), default="bastin")
+    ap.add_argument("--workload", choices=("bastin", "dos", "batch"), default="bastin")
+    ap.add_argument("--jobs", type=int, default=64, help="instances of the batch workload")
     ap.add_argument("--L", type=int, default=None, help="lattice size (cells per side)")
     ap.add_argument("--M", type=int, default=None)
     ap.add_argument("--R", type=int, default=None)
     ap.add_argument("--nump", type=int, default=2048)
+    ap.add_argument("--precision", choices=tuple(PRECISIONS), default="fp64",
+                    help="contraction precision of the 2D table (int8xS: S int8 digits on INT8 MFMA)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="only time a device-to-device copy of 1 GiB (the reference kernel for PMC byte counts)")
@@ -165,15 +233,41 @@ def main():
         dphi = lq.DeviceStates(ctx, phi)
         log(f"bastin N={n} M={M} R={R} nump={nump}")
 
-        def step():
-            fd = lq.kubo_fft(Hg, Vxg, [Vxg, Vyg], dphi, M, nump, lq.LQ_KUBO_BASTIN)
+        prec = PRECISIONS[args.precision]
+
+        def step(precision=prec):
+            fd = lq.kubo_fft(Hg, Vxg, [Vxg, Vyg], dphi, M, nump, lq.LQ_KUBO_BASTIN, precision=precision)
             return [lq.kubo_post(ctx, lq.LQ_KUBO_BASTIN, fd[o], n, 9.2, 0.0) for o in range(2)]
 
         moments_per_step = 2 * M * M * R
         config = {"workload": f"kubo_bastin_2d honeycomb-haldane {L}x{L} cells N={n} t2=0.1 W=2 "
                               f"M={M}x{M} R={R} sigma_xx+sigma_xy nump={nump}",
-                  "N": n, "M": M, "R": R, "nump": nump, "components": 2, "precision": "fp64"}
+                  "N": n, "M": M, "R": R, "nump": nump, "components": 2, "precision": args.precision}
+    elif args.workload == "batch":
+        if args.precision != "fp64":
+            sys.exit("bench.py: --precision applies to the 2D (bastin) workload")
+        specs, mats = batch_instances(lq, ctx, args.jobs)
+        log(f"batch: {len(specs)} jobs, sum N = {sum(m[1].n for m in mats)}")
+
+        def step():
+            b = lq.Batch(ctx, chunk=32)
+            ids = [b.submit(H, ds, sp["M"]) for sp, (H, _, ds) in zip(specs, mats)]
+            b.run()
+            out = [b.result(j) for j in ids]
+            b.close()
+            return out
+
+        moments_per_step = sum(sp["M"] * sp["R"] for sp in specs)
+        n = sum(m[1].n for m in mats)
+        M = max(sp["M"] for sp in specs)
+        R = max(sp["R"] for sp in specs)
+        config = {"workload": f"kpm_dos batch of {len(specs)}: cubic Anderson L16-96 + Kane-Mele N16k-1M, "
+                              f"M 256-2048, R 1-16 (seed 20261205)",
+                  "jobs": len(specs), "sum_N": n, "M": M, "R": R, "precision": "fp64",
+                  "instances": [f"{sp['kind']}{sp['L']} M{sp['M']} R{sp['R']}" for sp in specs]}
     else:
+        if args.precision != "fp64":
+            sys.exit("bench.py: --precision applies to the 2D (bastin) workload")
         L = args.L or 256
         M = args.M or 1024
         R = args.R or 8
@@ -199,11 +293,12 @@ def main():
         log(f"warmup {w + 1}/{args.warmup} done")
     ctx.synchronize()
     ctx.reset_timing()
+    result = None
     lat = []
     t_start = time.perf_counter()
     for k in range(args.steps):
         t0 = time.perf_counter()
-        step()
+        result = step()
         ctx.synchronize()
         lat.append((time.perf_counter() - t0) * 1e3)
         log(f"step {k + 1}/{args.steps}: {lat[-1]:.1f} ms")
@@ -232,7 +327,41 @@ def main():
             traffic_tab = t["bytes_per_launch"]
     except (OSError, ValueError, KeyError):
         traffic_tab = {}
-    if tm["contract_ms"] > tm["spmv_ms"]:
+    sequential = None
+    if args.workload == "batch":
+        # the same jobs, one after another through the single-job solver (untimed in value)
+        t0 = time.perf_counter()
+        seq = [lq.spectral_moments(H, None, ds, sp["M"]) for sp, (H, _, ds) in zip(specs, mats)]
+        ctx.synchronize()
+        t_seq = time.perf_counter() - t0
+        same = all(np.array_equal(a, b) for a, b in zip(seq, result))
+        sequential = {"ms": t_seq * 1e3, "speedup_of_batch": t_seq / (elapsed / args.steps),
+                      "bitwise_equal": bool(same)}
+        log(f"sequential single-job solves: {t_seq * 1e3:.1f} ms, batch identical: {same}")
+    accuracy = None
+    split = args.workload == "bastin" and args.precision != "fp64"
+    if split:
+        # sigma of the split-precision path against the FP64 path (untimed)
+        ref = step(0)
+        accuracy = {}
+        for o, comp in enumerate(("xx", "xy")):
+            s_ref, s_got = np.asarray(ref[o][1]), np.asarray(result[o][1])
+            accuracy[f"sigma_{comp}_rel_err"] = float(np.abs(s_got - s_ref).max() / np.abs(s_ref).max())
+        log(f"accuracy vs fp64: {accuracy}")
+    if split and tm["contract_ms"] > tm["spmv_ms"]:
+        S = PRECISIONS[args.precision]
+        launches = max(tm["contract_launches"], 1)
+        kp = -(-n * R // 4096) * 4096
+        ops = 2.0 * M * M * kp * 4 * (S * (S + 1) // 2)  # INT8 MFMA ops per launch
+        avg_ms = tm["contract_ms"] / launches
+        tops = ops / (avg_ms * 1e-3) / 1e12
+        kname = f"k_contract_i8<{S}, {2 if S <= 5 else 1}>"
+        roofline = {"bound": "mfma", "kernel": kname + " (+ k_slice)", "achieved": tops,
+                    "peak": INT8_MFMA_PEAK_TOPS, "unit": "TOP/s", "frac": tops / INT8_MFMA_PEAK_TOPS,
+                    "traffic": traffic_tab.get(kname),
+                    "per_launch": f"{ops:.4g} int8 op (8 M^2 ceil(N R / 4096) 4096 S(S+1)/2)",
+                    "avg_launch_ms": avg_ms}
+    elif tm["contract_ms"] > tm["spmv_ms"]:
         per_launch = tm["contract_flops"] / max(tm["contract_launches"], 1)
         kname = "k_contract_f64_lds" if min(config["M"], config["M"]) >= 128 else "k_contract_f64"
         roofline = {"bound": "mfma", "kernel": kname, "achieved": ctr_tfs, "peak": FP64_MFMA_PEAK_TFS,
@@ -252,6 +381,8 @@ def main():
         try:
             if args.workload == "bastin":
                 cpu = cpu_baseline_bastin(host_ops, M, R, nump, phi)
+            elif args.workload == "batch":
+                cpu = cpu_baseline_batch(specs, mats)
             else:
                 cpu = cpu_baseline_dos(Hh, M, R, phi)
         except Exception as e:  # the baseline is a report, not the measurement
@@ -268,7 +399,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "complex128",
+        "dtype": "complex128" if args.precision == "fp64" else f"complex128 in, {args.precision} MFMA contraction",
         "data": "synthetic",
         "config": config,
         "latency_ms": {"p50": _pct(lat, 50), "p99": _pct(lat, 99)},
@@ -276,6 +407,10 @@ def main():
         "kernels": kernels,
         "cpu_baseline": cpu,
     }
+    if accuracy is not None:
+        out["accuracy_vs_fp64"] = accuracy
+    if sequential is not None:
+        out["sequential_single_job"] = sequential
     print(json.dumps(out), flush=True)
 
 

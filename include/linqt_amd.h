@@ -57,11 +57,20 @@ enum lq_algo1d {
 
 /* Contraction precision of the 2D table. */
 enum lq_precision {
-  LQ_PREC_FP64 = 0 /* FP64 MFMA */
+  LQ_PREC_FP64 = 0,   /* FP64 MFMA */
+  /* Split precision: each operand is cut, per 4096-element group of a row,
+   * into S int8 digits under a shared power-of-two exponent; digit products
+   * of levels t + u <= S + 1 are summed exactly on INT8 MFMA (int32) and
+   * combined in FP64.  Error per product ~ (S + 1) 2^-7S of the group maxima. */
+  LQ_PREC_INT8X4 = 4, /* S = 4 */
+  LQ_PREC_INT8X5 = 5, /* S = 5 */
+  LQ_PREC_INT8X6 = 6, /* S = 6 */
+  LQ_PREC_INT8X7 = 7  /* S = 7 */
 };
 
 typedef struct lq_ctx_s *lq_ctx;
 typedef struct lq_csr_s *lq_csr;
+typedef struct lq_batch_s *lq_batch;
 
 /* Per-context kernel timing (HIP events on the context stream), summed since
  * the last lq_ctx_reset_timing.  Bytes / flops are algorithmic counts. */
@@ -118,6 +127,9 @@ int lq_spmv(lq_csr A, double a_re, double a_im, const double *x, double b_re, do
  * m0 < ML, m1 < MR, k < K.  All DEVICE pointers; FP64 MFMA. */
 int lq_compute_mom_table(lq_ctx ctx, const double *L, long long ldl, int ML, const double *R,
                          long long ldr, int MR, long long K, double *sub, int ldsub);
+/* The same table in a chosen lq_precision (LQ_PREC_FP64 equals lq_compute_mom_table). */
+int lq_compute_mom_table_prec(lq_ctx ctx, const double *L, long long ldl, int ML, const double *R,
+                              long long ldr, int MR, long long K, int precision, double *sub, int ldsub);
 
 /* ---- moment solvers (host phi in, host moments out) ---------------------- */
 /* Replaces chebyshev::SpectralMoments (src/chebyshev_solver.cpp:176-200).
@@ -167,6 +179,32 @@ int lq_greenwood_fftgrid(lq_ctx ctx, const double *mu, int M0, int M1, int n, do
 /* kuboBastinFromChebmom_FFTgrid.cpp:57-106: num_div-1 rows (cumulative). */
 int lq_bastin_fftgrid(lq_ctx ctx, const double *mu, int M0, int M1, int n, double W, double Ec,
                       int num_div, double *E, double *sigma);
+
+/* ---- batched spectral-moment jobs ------------------------------------------ */
+/* Many SpectralMoments jobs (OP = identity, src/chebyshev_solver.cpp:176-200),
+ * each with its own rescaled H, states and M, advanced together: one grouped
+ * launch per recursion step per vector-width class.  This is the submit/poll
+ * surface a serving loop drives (continuous batching): jobs submitted between
+ * lq_batch_run calls join at the next chunk boundary when their device
+ * footprint fits the budget.  A job's moments are bitwise identical alone,
+ * in any mix, and to lq_spectral_moments on a fresh mu (R <= 64). */
+int lq_batch_create(lq_ctx ctx, size_t hbm_budget /* 0: 80% of free */, int chunk /* steps, 0: 32 */,
+                    lq_batch *b);
+int lq_batch_destroy(lq_batch b);
+/* Device bytes a job holds while it runs (the admission footprint). */
+int lq_batch_job_bytes(int n, int R, int M, int chunk, size_t *bytes);
+/* Queue a job (1 <= R <= 64).  Host phi is copied; a device phi must stay
+ * valid until the job is admitted.  H must stay alive until it finishes. */
+int lq_batch_submit(lq_batch b, lq_csr H, const double *phi, int R, int M, int *job);
+/* Run chunks until nothing is queued or running (max_chunks <= 0) or for
+ * max_chunks chunks; reports the jobs still running and queued. */
+int lq_batch_run(lq_batch b, int max_chunks, int *running, int *queued);
+/* state: 0 queued, 1 running, 2 finished. */
+int lq_batch_poll(lq_batch b, int job, int *state);
+/* mu (M complex) of a finished job, which is then released. */
+int lq_batch_result(lq_batch b, int job, double *mu);
+/* Gather unroll the tuner chose for jobs with R states (0: still tuning). */
+int lq_batch_unroll(lq_batch b, int R, int *unroll);
 
 #ifdef __cplusplus
 }

@@ -28,13 +28,14 @@ import numpy as np
 
 from . import _lib
 from ._lib import (LQ_ALGO_DIRECT, LQ_ALGO_DOUBLING, LQ_FILL_GENERAL, LQ_FILL_HERMITIAN_UPPER,
-                   LQ_KUBO_BASTIN, LQ_KUBO_GREENWOOD, LQ_PREC_FP64, LinqtError)
+                   LQ_KUBO_BASTIN, LQ_KUBO_GREENWOOD, LQ_PREC_FP64, LQ_PREC_INT8X4, LQ_PREC_INT8X5,
+                   LQ_PREC_INT8X6, LQ_PREC_INT8X7, LinqtError)
 
 __all__ = [
-    "Context", "SparseMatrix", "DeviceStates", "spectral_moments", "correlation_expansion_moments", "kubo_fft", "kubo_post",
+    "Context", "SparseMatrix", "DeviceStates", "Batch", "spectral_moments", "correlation_expansion_moments", "kubo_fft", "kubo_post",
     "jackson_1d", "jackson_2d", "dos_fftgrid", "greenwood_fftgrid", "bastin_fftgrid", "LinqtError",
     "LQ_FILL_HERMITIAN_UPPER", "LQ_FILL_GENERAL", "LQ_KUBO_GREENWOOD", "LQ_KUBO_BASTIN", "LQ_ALGO_DIRECT",
-    "LQ_ALGO_DOUBLING", "LQ_PREC_FP64",
+    "LQ_ALGO_DOUBLING", "LQ_PREC_FP64", "LQ_PREC_INT8X4", "LQ_PREC_INT8X5", "LQ_PREC_INT8X6", "LQ_PREC_INT8X7",
 ]
 
 
@@ -237,6 +238,70 @@ def spectral_moments(H: SparseMatrix, OP, phi: np.ndarray, M: int, algo: int = L
     mu = np.zeros(M, dtype=np.complex128) if mu is None else np.array(mu, dtype=np.complex128)
     _lib.call("lq_spectral_moments", H.h, _h(OP), pp, R, M, algo, _dp(mu.view(np.float64)))
     return mu
+
+
+class Batch:
+    """Batched ``SpectralMoments`` jobs (identity OP) on one context.
+
+    ``submit`` queues a job; ``run`` advances every admitted job by chunks of
+    recursion steps with one grouped launch per step per state-count class,
+    admitting queued jobs at chunk boundaries while their device footprint
+    fits ``hbm_budget`` bytes (0: 80 % of free memory).  A job's moments are
+    bitwise the same alone, in any mix, and to ``spectral_moments``."""
+
+    def __init__(self, ctx: Context, hbm_budget: int = 0, chunk: int = 32):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        self.chunk = chunk
+        _lib.call("lq_batch_create", ctx.h, hbm_budget, chunk, C.byref(self.h))
+        self._keep = {}
+
+    @staticmethod
+    def job_bytes(n: int, R: int, M: int, chunk: int = 32) -> int:
+        b = C.c_size_t(0)
+        _lib.call("lq_batch_job_bytes", n, R, M, chunk, C.byref(b))
+        return b.value
+
+    def submit(self, H: SparseMatrix, phi, M: int) -> int:
+        pp, R, keep = _states(phi)
+        job = C.c_int(0)
+        _lib.call("lq_batch_submit", self.h, H.h, pp, R, M, C.byref(job))
+        self._keep[job.value] = (H, M, phi if isinstance(phi, DeviceStates) else None)
+        return job.value
+
+    def run(self, max_chunks: int = 0):
+        """Returns (running, queued) after at most max_chunks chunks (0: until idle)."""
+        r, q = C.c_int(0), C.c_int(0)
+        _lib.call("lq_batch_run", self.h, max_chunks, C.byref(r), C.byref(q))
+        return r.value, q.value
+
+    def state(self, job: int) -> int:
+        s = C.c_int(0)
+        _lib.call("lq_batch_poll", self.h, job, C.byref(s))
+        return s.value
+
+    def result(self, job: int) -> np.ndarray:
+        M = self._keep[job][1]
+        mu = np.zeros(M, dtype=np.complex128)
+        _lib.call("lq_batch_result", self.h, job, _dp(mu.view(np.float64)))
+        del self._keep[job]
+        return mu
+
+    def unroll(self, R: int) -> int:
+        u = C.c_int(0)
+        _lib.call("lq_batch_unroll", self.h, R, C.byref(u))
+        return u.value
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value and self.ctx.h is not None:
+            _lib.call("lq_batch_destroy", self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def correlation_expansion_moments(H: SparseMatrix, OPL, OPR, phi: np.ndarray, M0: int, M1: int | None = None,

@@ -25,6 +25,10 @@ LQ_KUBO_BASTIN = 1
 LQ_ALGO_DIRECT = 0
 LQ_ALGO_DOUBLING = 1
 LQ_PREC_FP64 = 0
+LQ_PREC_INT8X4 = 4
+LQ_PREC_INT8X5 = 5
+LQ_PREC_INT8X6 = 6
+LQ_PREC_INT8X7 = 7
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -66,6 +70,16 @@ _SIGS = {
     "lq_spmv": (C.c_int, [_vp, C.c_double, C.c_double, _vp, C.c_double, C.c_double, _vp, C.c_int]),
     "lq_compute_mom_table": (C.c_int, [_vp, _vp, C.c_longlong, C.c_int, _vp, C.c_longlong, C.c_int, C.c_longlong,
                                        _vp, C.c_int]),
+    "lq_compute_mom_table_prec": (C.c_int, [_vp, _vp, C.c_longlong, C.c_int, _vp, C.c_longlong, C.c_int,
+                                            C.c_longlong, C.c_int, _vp, C.c_int]),
+    "lq_batch_create": (C.c_int, [_vp, C.c_size_t, C.c_int, C.POINTER(_vp)]),
+    "lq_batch_destroy": (C.c_int, [_vp]),
+    "lq_batch_job_bytes": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_size_t)]),
+    "lq_batch_submit": (C.c_int, [_vp, _vp, _dp, C.c_int, C.c_int, _ip]),
+    "lq_batch_run": (C.c_int, [_vp, C.c_int, _ip, _ip]),
+    "lq_batch_poll": (C.c_int, [_vp, C.c_int, _ip]),
+    "lq_batch_result": (C.c_int, [_vp, C.c_int, _dp]),
+    "lq_batch_unroll": (C.c_int, [_vp, C.c_int, _ip]),
     "lq_spectral_moments": (C.c_int, [_vp, _vp, _dp, C.c_int, C.c_int, C.c_int, _dp]),
     "lq_correlation_moments": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(_vp), _dp, C.c_int, C.c_int, C.c_int,
                                          C.c_int, C.c_int, _dp]),

@@ -16,17 +16,6 @@ thread_local std::string g_last_error;
 
 constexpr double kPi = 3.14159265358979323846;
 
-struct DeviceGuard {
-  int prev = -1;
-  explicit DeviceGuard(int dev) {
-    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-    if (prev != dev) (void)hipSetDevice(dev);
-  }
-  ~DeviceGuard() {
-    if (prev >= 0) (void)hipSetDevice(prev);
-  }
-};
-
 inline double2 d2(double x, double y) { return make_double2(x, y); }
 
 }  // namespace
@@ -244,6 +233,8 @@ int spmm(lq_ctx ctx, lq_csr A, double2 alpha, double2 beta, const VecSet *sets, 
   return time_end(ctx, kCatSpmv, ev, (double)spmv_bytes(A, k, sets[0].z != nullptr));
 }
 
+}  // namespace
+
 bool is_device_ptr(const void *p, int device) {
   hipPointerAttribute_t attr;
   if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
@@ -267,6 +258,8 @@ int upload_states(lq_ctx ctx, const double *phi, int n, int r0, int g, double2 *
   return LQ_OK;
 }
 
+namespace {
+
 size_t free_device_bytes() {
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
@@ -288,21 +281,57 @@ struct Pipe2D {
   const lq_csr *OPR;
   int n, ML, MR;
   double2 *raw;  // nopr tables of ML x MR (ld MR)
+  int precision = LQ_PREC_FP64;
 };
+
+// out (+)= A^H B for A: MA rows (ld lda), B: MB rows (ld ldb), K columns, in
+// the given precision.  For the split-precision path A's digit planes are
+// reused when slice_a is false (same A as the previous call on this context).
+int contract_table(lq_ctx ctx, int precision, const double2 *A, long long lda, int MA, const double2 *B,
+                   long long ldb, int MB, long long K, double2 *out, int ldo, bool accumulate, bool slice_a) {
+  hipEvent_t ev;
+  double2 *work;
+  size_t wbytes = 0;
+  int nsplit = 1;
+  if (precision == LQ_PREC_FP64) {
+    LQ_TRY(contract_workspace_bytes(MA, MB, K, &wbytes, &nsplit));
+    LQ_TRY(ctx_buf(ctx, "cwork", wbytes, (void **)&work));
+    LQ_TRY(time_begin(ctx, kCatContract, &ev));
+    LQ_TRY(launch_contract_f64(ctx->stream, A, lda, MA, B, ldb, MB, K, work, nsplit));
+  } else {
+    const int S = precision;
+    size_t qa, ea, qb, eb;
+    LQ_TRY(ozaki_slice_bytes(MA, K, S, &qa, &ea));
+    LQ_TRY(ozaki_slice_bytes(MB, K, S, &qb, &eb));
+    signed char *QA, *QB;
+    int *EA, *EB;
+    LQ_TRY(ctx_buf(ctx, "ozQA", qa, (void **)&QA));
+    LQ_TRY(ctx_buf(ctx, "ozEA", ea, (void **)&EA));
+    LQ_TRY(ctx_buf(ctx, "ozQB", qb, (void **)&QB));
+    LQ_TRY(ctx_buf(ctx, "ozEB", eb, (void **)&EB));
+    nsplit = ozaki_nsplit(MA, MB, K);
+    wbytes = (size_t)nsplit * MA * MB * sizeof(double2);
+    LQ_TRY(ctx_buf(ctx, "cwork", wbytes, (void **)&work));
+    LQ_TRY(time_begin(ctx, kCatContract, &ev));
+    if (slice_a) LQ_TRY(launch_slice(ctx->stream, A, lda, MA, K, S, QA, EA));
+    LQ_TRY(launch_slice(ctx->stream, B, ldb, MB, K, S, QB, EB));
+    LQ_TRY(launch_contract_i8(ctx->stream, QA, EA, MA, QB, EB, MB, K, S, work, nsplit));
+  }
+  LQ_TRY(time_end(ctx, kCatContract, ev, 8.0 * MA * (double)MB * (double)K));
+  LQ_TRY(time_begin(ctx, kCatPost, &ev));
+  LQ_TRY(launch_contract_reduce(ctx->stream, work, nsplit, MA, MB, out, ldo, accumulate ? 1 : 0));
+  return time_end(ctx, kCatPost, ev, 0);
+}
 
 int pipe2d_group(Pipe2D &P, const double *phi, int r0, int g, bool accumulate) {
   lq_ctx ctx = P.ctx;
   const int n = P.n;
   const size_t vec = (size_t)n * g;
-  double2 *phi_d, *Ls, *Rw, *Rs, *work;
+  double2 *phi_d, *Ls, *Rw, *Rs;
   LQ_TRY(ctx_buf(ctx, "phi", vec * sizeof(double2), (void **)&phi_d));
   LQ_TRY(ctx_buf(ctx, "Lstore", vec * (size_t)P.ML * sizeof(double2), (void **)&Ls));
   LQ_TRY(ctx_buf(ctx, "Rwork", 2 * vec * sizeof(double2), (void **)&Rw));
   LQ_TRY(ctx_buf(ctx, "Rstore", vec * (size_t)P.MR * P.nopr * sizeof(double2), (void **)&Rs));
-  size_t wbytes = 0;
-  int nsplit = 1;
-  LQ_TRY(contract_workspace_bytes(P.ML, P.MR, (long long)vec, &wbytes, &nsplit));
-  LQ_TRY(ctx_buf(ctx, "cwork", wbytes, (void **)&work));
   LQ_TRY(upload_states(ctx, phi, n, r0, g, phi_d));
 
   const double2 one = d2(1, 0), zero = d2(0, 0), two = d2(2, 0), mone = d2(-1, 0);
@@ -354,23 +383,16 @@ int pipe2d_group(Pipe2D &P, const double *phi, int r0, int g, bool accumulate) {
     }
   }
   // raw_o (+)= L^H R^o
-  for (int o = 0; o < P.nopr; ++o) {
-    hipEvent_t ev;
-    LQ_TRY(time_begin(ctx, kCatContract, &ev));
-    LQ_TRY(launch_contract_f64(ctx->stream, Ls, (long long)vec, P.ML, RS(o, 0), (long long)vec, P.MR, (long long)vec,
-                               work, nsplit));
-    LQ_TRY(time_end(ctx, kCatContract, ev, 8.0 * P.ML * (double)P.MR * (double)vec));
-    LQ_TRY(time_begin(ctx, kCatPost, &ev));
-    LQ_TRY(launch_contract_reduce(ctx->stream, work, nsplit, P.ML, P.MR, P.raw + (size_t)o * P.ML * P.MR, P.MR,
-                                  accumulate ? 1 : 0));
-    LQ_TRY(time_end(ctx, kCatPost, ev, 0));
-  }
+  for (int o = 0; o < P.nopr; ++o)
+    LQ_TRY(contract_table(ctx, P.precision, Ls, (long long)vec, P.ML, RS(o, 0), (long long)vec, P.MR, (long long)vec,
+                          P.raw + (size_t)o * P.ML * P.MR, P.MR, accumulate, o == 0));
   return LQ_OK;
 }
 
 // States per group: as many as fit (vectors per SpMM row <= 64 in total).
-int choose_group(int R, int n, int ML, int MR, int nopr) {
-  const size_t per_state = ((size_t)ML + (size_t)MR * nopr + 3) * (size_t)n * sizeof(double2);
+int choose_group(int R, int n, int ML, int MR, int nopr, int precision) {
+  size_t per_state = ((size_t)ML + (size_t)MR * nopr + 3) * (size_t)n * sizeof(double2);
+  if (precision != LQ_PREC_FP64) per_state += ((size_t)ML + MR) * 2 * precision * (size_t)n;  // digit planes
   size_t budget = free_device_bytes();
   budget = budget > ((size_t)4 << 30) ? budget - ((size_t)4 << 30) : budget / 2;  // keep head room
   int g = (int)std::min<size_t>((size_t)R, per_state ? budget / per_state : (size_t)R);
@@ -380,7 +402,7 @@ int choose_group(int R, int n, int ML, int MR, int nopr) {
 }
 
 int run_pipe2d(Pipe2D &P, const double *phi, int R) {
-  const int g = choose_group(R, P.n, P.ML, P.MR, P.nopr);
+  const int g = choose_group(R, P.n, P.ML, P.MR, P.nopr, P.precision);
   for (int r0 = 0; r0 < R; r0 += g) {
     const int gg = std::min(g, R - r0);
     LQ_TRY(pipe2d_group(P, phi, r0, gg, r0 > 0));
@@ -664,25 +686,26 @@ int lq_spmv(lq_csr A, double a_re, double a_im, const double *x, double b_re, do
   return spmm(ctx, A, d2(a_re, a_im), d2(b_re, b_im), &s, 1);
 }
 
-int lq_compute_mom_table(lq_ctx ctx, const double *L, long long ldl, int ML, const double *R, long long ldr, int MR,
-                         long long K, double *sub, int ldsub) {
+int lq_compute_mom_table_prec(lq_ctx ctx, const double *L, long long ldl, int ML, const double *R, long long ldr,
+                              int MR, long long K, int precision, double *sub, int ldsub) {
   if (!ctx || !L || !R || !sub || ML <= 0 || MR <= 0 || K <= 0 || ldl < K || ldr < K || ldsub < MR) {
     set_error("lq_compute_mom_table: invalid arguments");
     return LQ_ERR_ARG;
   }
+  if (!precision_ok(precision)) {
+    set_error("lq_compute_mom_table: unknown precision %d", precision);
+    return LQ_ERR_ARG;
+  }
   DeviceGuard dg(ctx->device);
-  size_t wbytes = 0;
-  int nsplit = 1;
-  LQ_TRY(contract_workspace_bytes(ML, MR, K, &wbytes, &nsplit));
-  double2 *work;
-  LQ_TRY(ctx_buf(ctx, "cwork", wbytes, (void **)&work));
-  hipEvent_t ev;
-  LQ_TRY(time_begin(ctx, kCatContract, &ev));
-  LQ_TRY(launch_contract_f64(ctx->stream, (const double2 *)L, ldl, ML, (const double2 *)R, ldr, MR, K, work, nsplit));
-  LQ_TRY(time_end(ctx, kCatContract, ev, 8.0 * ML * (double)MR * (double)K));
-  LQ_TRY(launch_contract_reduce(ctx->stream, work, nsplit, ML, MR, (double2 *)sub, ldsub, 0));
+  LQ_TRY(contract_table(ctx, precision, (const double2 *)L, ldl, ML, (const double2 *)R, ldr, MR, K, (double2 *)sub,
+                        ldsub, false, true));
   LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return collect_timing(ctx);
+}
+
+int lq_compute_mom_table(lq_ctx ctx, const double *L, long long ldl, int ML, const double *R, long long ldr, int MR,
+                         long long K, double *sub, int ldsub) {
+  return lq_compute_mom_table_prec(ctx, L, ldl, ML, R, ldr, MR, K, LQ_PREC_FP64, sub, ldsub);
 }
 
 // ---- 1D moments ----------------------------------------------------------------
@@ -833,7 +856,7 @@ int lq_correlation_moments(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, co
     set_error("lq_correlation_moments: invalid arguments");
     return LQ_ERR_ARG;
   }
-  if (precision != LQ_PREC_FP64) {
+  if (!precision_ok(precision)) {
     set_error("lq_correlation_moments: unknown precision %d", precision);
     return LQ_ERR_ARG;
   }
@@ -847,7 +870,7 @@ int lq_correlation_moments(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, co
   double2 *raw, *init = nullptr, *outd;
   LQ_TRY(ctx_buf(ctx, "raw", tab * nopr * sizeof(double2), (void **)&raw));
   LQ_TRY(ctx_buf(ctx, "mu_out", (size_t)M0 * M1 * sizeof(double2), (void **)&outd));
-  Pipe2D P{ctx, H, OPL, nopr, OPR, H->n, Mt, Mt, raw};
+  Pipe2D P{ctx, H, OPL, nopr, OPR, H->n, Mt, Mt, raw, precision};
   LQ_TRY(run_pipe2d(P, phi, R));
   for (int o = 0; o < nopr; ++o) {
     if (!padded) {
@@ -883,7 +906,7 @@ int lq_kubo_fft(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, const double 
     set_error("lq_kubo_fft: unknown formula %d", formula);
     return LQ_ERR_ARG;
   }
-  if (precision != LQ_PREC_FP64) {
+  if (!precision_ok(precision)) {
     set_error("lq_kubo_fft: unknown precision %d", precision);
     return LQ_ERR_ARG;
   }
@@ -911,7 +934,7 @@ int lq_kubo_fft(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, const double 
   if (rc == LQ_OK) rc = ctx_buf(ctx, "kubo_fd", (size_t)2 * nump * nopr * sizeof(double2), (void **)&fd);
   if (rc == LQ_OK) rc = ctx_buf(ctx, "kubo_coef", (size_t)M * sizeof(double), (void **)&coef);
   if (rc == LQ_OK) {
-    Pipe2D P{ctx, H, OPL, nopr, opr.data(), n, M, M, raw};
+    Pipe2D P{ctx, H, OPL, nopr, opr.data(), n, M, M, raw, precision};
     rc = run_pipe2d(P, phi, R);
   }
   if (rc == LQ_OK) {

@@ -67,6 +67,18 @@ enum DotMode {
   kDotDoubling = 2  // partials = sum conj(x_i)x_i and sum conj(x_i) y_i (row's own x)
 };
 
+// One job of a batched recursion launch (device table entry).
+struct BatchJobDev {
+  const int *sptr;
+  const int *col;
+  const double2 *val;
+  const double2 *phi;  // [n][k] states
+  double2 *buf0, *buf1;  // T_s ring
+  double2 *partial;      // [chunk slot][nblk][k-width] dot partials
+  double2 *mom;          // [M][k-width] per-state moments
+  int n, k, t0, steps, blk0, nblk;
+};
+
 // Launchers (lq_spmv.hip).  Return LQ_OK or an error code.
 int launch_spmm(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, const VecSet *sets,
                 int nsets, int dot_mode, const double2 *phi, double2 *partial, int *grid_used);
@@ -76,6 +88,12 @@ int launch_dot(hipStream_t st, int n, int k, const double2 *a, const double2 *b,
 int launch_reduce_partials(hipStream_t st, const double2 *partial, int nslots, int grid, int width,
                            double2 *out);
 int launch_copy(hipStream_t st, double2 *dst, const double2 *src, size_t count);
+// Batched recursion (one width class kp, grid = sum of the jobs' nblk): step
+// slot of the chunk whose first launch index is *tc.
+int launch_spmm_batch(hipStream_t st, int kp, int unroll, const BatchJobDev *jobs, int njobs, int grid,
+                      const int *tc, int slot);
+int launch_batch_reduce(hipStream_t st, const BatchJobDev *jobs, int njobs, const int *tc, int nslots, int kp);
+int launch_set_int(hipStream_t st, int *p, int v);
 // dst[i*g + v] = src[(r0 + v)*n + i] (state-major device states -> [n][g]).
 int launch_interleave(hipStream_t st, const double2 *src, int n, int r0, int g, double2 *dst);
 
@@ -87,6 +105,19 @@ int launch_contract_f64(hipStream_t st, const double2 *A, long long lda, int M0,
                         long long ldb, int M1, long long K, double2 *work, int nsplit);
 int launch_contract_reduce(hipStream_t st, const double2 *work, int nsplit, int M0, int M1, double2 *out,
                            int ldo, int accumulate);
+
+// Split-precision contraction (lq_ozaki.hip).  launch_slice cuts rows x K
+// complex values (row stride ldx) into S int8 digit planes Q[row][t][re/im][Kp]
+// and per-group exponents E[row][Kp / 4096]; launch_contract_i8 writes nsplit
+// partial tables to work (same layout as launch_contract_f64).
+long long ozaki_kp(long long K);
+int ozaki_slice_bytes(int rows, long long K, int S, size_t *qbytes, size_t *ebytes);
+int ozaki_nsplit(int M0, int M1, long long K);
+int launch_slice(hipStream_t st, const double2 *X, long long ldx, int rows, long long K, int S, signed char *Q,
+                 int *E);
+int launch_contract_i8(hipStream_t st, const signed char *QA, const int *EA, int M0, const signed char *QB,
+                       const int *EB, int M1, long long K, int S, double2 *work, int nsplit);
+inline bool precision_ok(int p) { return p == 0 || (p >= 4 && p <= 7); }
 
 // Post-processing (lq_post.hip).
 int launch_corr_epilogue(hipStream_t st, const double2 *raw, int ldr, const double2 *init, int M0,
@@ -143,6 +174,21 @@ struct lq_csr_s {
 };
 
 namespace lq {
+// Makes a context's device current for the scope of a call.
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+bool is_device_ptr(const void *p, int device);
+// State-major phi (host, or device memory of the context's GPU) -> device
+// [n][g] interleaved block of states r0..r0+g-1.
+int upload_states(lq_ctx ctx, const double *phi, int n, int r0, int g, double2 *dst);
 // Context helpers (lq_api.hip).
 int ctx_buf(lq_ctx ctx, const char *name, size_t bytes, void **p);
 int time_begin(lq_ctx ctx, int cat, hipEvent_t *ev);

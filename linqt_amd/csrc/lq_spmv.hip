@@ -33,7 +33,7 @@ __device__ __forceinline__ double2 conj_mul(double2 a, double2 b) {
 }
 
 template <int KP, int NDOT>
-__device__ __forceinline__ void block_reduce_store(double2 (&d)[2], double2 *partial) {
+__device__ __forceinline__ void block_reduce_store(double2 (&d)[2], double2 *partial, int slot) {
   __shared__ double2 sh[2][256];
   const int tid = threadIdx.x;
 #pragma unroll
@@ -52,7 +52,7 @@ __device__ __forceinline__ void block_reduce_store(double2 (&d)[2], double2 *par
   }
   if (tid < KP) {
 #pragma unroll
-    for (int q = 0; q < NDOT; ++q) partial[((size_t)blockIdx.x * NDOT + q) * KP + tid] = sh[q][tid];
+    for (int q = 0; q < NDOT; ++q) partial[((size_t)slot * NDOT + q) * KP + tid] = sh[q][tid];
   }
 }
 
@@ -64,13 +64,14 @@ constexpr int kXcds = 8;
 
 __device__ __forceinline__ int xcd_group(int b, int q) { return (b % kXcds) * q + b / kXcds; }
 
+// Rows of row groups g0, g0 + gstep, ... (RPB = 256 / KP rows each) of one
+// launch; dot contributions are accumulated into d.
 template <int KP, int DOT, int kUnroll>
-__global__ __launch_bounds__(256) void k_spmm(int n, const int *__restrict__ sptr, const int *__restrict__ col,
-                                              const double2 *__restrict__ val, double2 alpha, double2 beta,
-                                              SetArgs sa, int nrowgroups, const double2 *__restrict__ phi,
-                                              double2 *__restrict__ partial, int q) {
+__device__ __forceinline__ void spmm_body(int n, const int *__restrict__ sptr, const int *__restrict__ col,
+                                          const double2 *__restrict__ val, double2 alpha, double2 beta,
+                                          const SetArgs &sa, int nrowgroups, const double2 *__restrict__ phi,
+                                          int g0, int gstep, double2 (&d)[2]) {
   constexpr int RPB = 256 / KP;
-  constexpr int NDOT = (DOT == kDotDoubling) ? 2 : 1;
   const int tid = threadIdx.x;
   const int v = tid % KP;
   const int set = (v < sa.k[0]) ? 0 : 1;
@@ -80,11 +81,6 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int *__restrict__ spt
   const double2 *__restrict__ x = sa.x[set];
   const double2 *z = sa.z[set];
   double2 *y = sa.y[set];
-  double2 d[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
-
-  // q > 0: one row group per block, XCD-remapped; q == 0: grid-stride loop
-  const int g0 = q > 0 ? xcd_group(blockIdx.x, q) : blockIdx.x;
-  const int gstep = q > 0 ? nrowgroups : gridDim.x;
   for (int g = g0; g < nrowgroups; g += gstep) {
     const int row = g * RPB + tid / KP;
     if (row < n && lane_on) {
@@ -140,8 +136,77 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int *__restrict__ spt
       }
     }
   }
-  if (DOT != kDotNone) block_reduce_store<KP, NDOT>(d, partial);
 }
+
+template <int KP, int DOT, int kUnroll>
+__global__ __launch_bounds__(256) void k_spmm(int n, const int *__restrict__ sptr, const int *__restrict__ col,
+                                              const double2 *__restrict__ val, double2 alpha, double2 beta,
+                                              SetArgs sa, int nrowgroups, const double2 *__restrict__ phi,
+                                              double2 *__restrict__ partial, int q) {
+  constexpr int NDOT = (DOT == kDotDoubling) ? 2 : 1;
+  double2 d[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  // q > 0: one row group per block, XCD-remapped; q == 0: grid-stride loop
+  const int g0 = q > 0 ? xcd_group(blockIdx.x, q) : blockIdx.x;
+  const int gstep = q > 0 ? nrowgroups : gridDim.x;
+  spmm_body<KP, DOT, kUnroll>(n, sptr, col, val, alpha, beta, sa, nrowgroups, phi, g0, gstep, d);
+  if (DOT != kDotNone) block_reduce_store<KP, NDOT>(d, partial, blockIdx.x);
+}
+
+// Batched recursion step over many independent jobs (one vector-width class):
+// block b belongs to the job j with blk0_j <= b < blk0_j + nblk_j and runs that
+// job's step s = t - t0_j with the same row schedule, dot fusion and partial
+// layout as a single-job launch with nblk_j blocks, so every job's moments are
+// independent of what else is in the batch.
+//   s == 0:  T_1 = H~ T_0                     (x = buf0, y = buf1)
+//   s >= 1:  T_{s+1} = 2 H~ T_s - T_{s-1}     (in place over T_{s-1})
+template <int KP, int kUnroll>
+__global__ __launch_bounds__(256) void k_spmm_batch(const BatchJobDev *__restrict__ jobs, int njobs,
+                                                    const int *__restrict__ tc, int slot) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].blk0 <= (int)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const BatchJobDev &J = jobs[lo];
+  const int s = tc[0] + slot - J.t0;
+  if (s < 0 || s >= J.steps) return;  // whole block leaves together
+  const int lb = blockIdx.x - J.blk0;
+  SetArgs sa{};
+  const bool odd = s & 1;
+  sa.x[0] = sa.x[1] = odd ? J.buf1 : J.buf0;
+  double2 *y = odd ? J.buf0 : J.buf1;
+  sa.y[0] = sa.y[1] = y;
+  sa.z[0] = sa.z[1] = s == 0 ? nullptr : y;
+  sa.k[0] = J.k;
+  sa.k[1] = 0;
+  const double2 alpha = s == 0 ? make_double2(1.0, 0.0) : make_double2(2.0, 0.0);
+  const double2 beta = s == 0 ? make_double2(0.0, 0.0) : make_double2(-1.0, 0.0);
+  constexpr int RPB = 256 / KP;
+  const int groups = (J.n + RPB - 1) / RPB;
+  double2 d[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  spmm_body<KP, kDotPhi, kUnroll>(J.n, J.sptr, J.col, J.val, alpha, beta, sa, groups, J.phi, lb, J.nblk, d);
+  block_reduce_store<KP, 1>(d, J.partial + (size_t)slot * J.nblk * KP, lb);
+}
+
+// mom[(s + 1) * KP + v] = sum_b partial[slot][b][v] for every step s a job ran
+// in this chunk (the same block order as k_reduce_partials).
+__global__ void k_batch_reduce(const BatchJobDev *__restrict__ jobs, const int *__restrict__ tc, int nslots, int kp) {
+  const BatchJobDev &J = jobs[blockIdx.y];
+  const int slot = blockIdx.x;
+  const int v = threadIdx.x;
+  const int s = tc[0] + slot - J.t0;
+  if (s < 0 || s >= J.steps || v >= kp) return;
+  const double2 *p = J.partial + (size_t)slot * J.nblk * kp + v;
+  double sx = 0.0, sy = 0.0;
+  for (int b = 0; b < J.nblk; ++b) {
+    sx += p[(size_t)b * kp].x;
+    sy += p[(size_t)b * kp].y;
+  }
+  J.mom[(size_t)(s + 1) * kp + v] = make_double2(sx, sy);
+}
+
+__global__ void k_set_int(int *p, int v) { *p = v; }
 
 // Lane-per-row variant: one lane owns one row and all KS vectors of each of
 // NSETS sets (registers), so each (col, value) pair is loaded once and the
@@ -214,7 +279,7 @@ __global__ __launch_bounds__(256) void k_dot(int n, int k, const double2 *__rest
       d[0].y += t.y;
     }
   }
-  block_reduce_store<KP, 1>(d, partial);
+  block_reduce_store<KP, 1>(d, partial, blockIdx.x);
 }
 
 __global__ void k_reduce_partials(const double2 *__restrict__ partial, int nslots, int grid, int width,
@@ -395,6 +460,42 @@ int launch_spmm(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, c
       set_error("launch_spmm: %d vectors per row exceed 64", ktot);
       return LQ_ERR_ARG;
   }
+}
+
+int launch_spmm_batch(hipStream_t st, int kp, int unroll, const BatchJobDev *jobs, int njobs, int grid,
+                      const int *tc, int slot) {
+  if (njobs <= 0 || grid <= 0) return LQ_OK;
+#define LQ_BU(KPV, UV)                                                                                     \
+  if (kp == KPV && unroll == UV) {                                                                       \
+    hipLaunchKernelGGL((k_spmm_batch<KPV, UV>), dim3(grid), dim3(256), 0, st, jobs, njobs, tc, slot); \
+    LQ_HIP_CHECK(hipGetLastError());                                                                     \
+    return LQ_OK;                                                                                        \
+  }
+#define LQ_BK(KPV) LQ_BU(KPV, 4) LQ_BU(KPV, 8) LQ_BU(KPV, 12) LQ_BU(KPV, 16)
+  LQ_BK(1)
+  LQ_BK(2)
+  LQ_BK(4)
+  LQ_BK(8)
+  LQ_BK(16)
+  LQ_BK(32)
+  LQ_BK(64)
+#undef LQ_BK
+#undef LQ_BU
+  set_error("launch_spmm_batch: unsupported width %d / unroll %d", kp, unroll);
+  return LQ_ERR_ARG;
+}
+
+int launch_batch_reduce(hipStream_t st, const BatchJobDev *jobs, int njobs, const int *tc, int nslots, int kp) {
+  if (njobs <= 0 || nslots <= 0) return LQ_OK;
+  hipLaunchKernelGGL(k_batch_reduce, dim3(nslots, njobs), dim3(kp < 64 ? 64 : kp), 0, st, jobs, tc, nslots, kp);
+  LQ_HIP_CHECK(hipGetLastError());
+  return LQ_OK;
+}
+
+int launch_set_int(hipStream_t st, int *p, int v) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, p, v);
+  LQ_HIP_CHECK(hipGetLastError());
+  return LQ_OK;
 }
 
 int launch_dot(hipStream_t st, int n, int k, const double2 *a, const double2 *b, double2 *partial, int *grid_used) {

@@ -11,7 +11,10 @@ systems of the benchmarks, deterministically from a seed:
 * honeycomb L x L cells (2 sites), hopping -t, Haldane second-neighbour
   t2 e^{+-i phi}, disorder U[-W/2, W/2], plus the velocity operators
   V_a = i[H, X_a] (minimum-image bond vectors);
-* cubic lattice L^3, hopping -t, disorder U[-W/2, W/2].
+* cubic lattice L^3, hopping -t, disorder U[-W/2, W/2];
+* Kane-Mele-like honeycomb, 4 orbitals per cell (2 sites x 2 spins):
+  hopping -t, intrinsic spin-orbit i lso nu_ij s_z on second neighbours,
+  Rashba i lr (s x d)_z on first neighbours, disorder U[-W/2, W/2].
 
 Random numbers come from a counter-based splitmix64 (stream, index) ->
 uint64 -> U[0, 1) with 53 bits, so every array element is independent of
@@ -162,6 +165,66 @@ def haldane_honeycomb(L: int, t: float = 1.0, t2: float = 0.1, phi: float = np.p
     Vx = _to_upper_csr(n, rows[offd], cols[offd], 1j * disp[offd, 0] * vals[offd])
     Vy = _to_upper_csr(n, rows[offd], cols[offd], 1j * disp[offd, 1] * vals[offd])
     return H, _drop_zeros(Vx), _drop_zeros(Vy)
+
+
+def kane_mele(L: int, t: float = 1.0, lso: float = 0.06, lr: float = 0.05, W: float = 1.0,
+              seed: int = 20261206) -> CSR:
+    """Honeycomb L x L cells, 2 sites x 2 spins (n = 4 L^2), periodic; upper CSR.
+
+    Geometry as haldane_honeycomb; index 4*(m + L*k) + 2*s + sigma.  The
+    second-neighbour entry H(i, i + b) is i lso nu sigma_z with nu = +1 on A
+    and -1 on B (the Haldane pattern at phi = pi/2, spin-resolved); the
+    Rashba block of a first-neighbour bond with unit vector d is
+    i lr (sigma_x d_y - sigma_y d_x).
+    """
+    if L < 3:
+        raise ValueError("L >= 3 keeps periodic neighbours distinct")
+    a1 = np.array([1.0, 0.0])
+    a2 = np.array([0.5, np.sqrt(3.0) / 2.0])
+    dAB = (a1 + a2) / 3.0
+    ncell = L * L
+    n = 4 * ncell
+    c = np.arange(ncell)
+    m, k = c % L, c // L
+
+    def site(mm, kk, s, sg):
+        return 4 * ((mm % L) + L * (kk % L)) + 2 * s + sg
+
+    rows, cols, vals = [], [], []
+
+    def add(i, j, v):
+        rows.append(i)
+        cols.append(j)
+        vals.append(np.broadcast_to(np.asarray(v, dtype=np.complex128), i.shape).copy())
+
+    eps = W * (uniform(seed, 1, 2 * ncell) - 0.5)  # one value per site, both spins
+    for s in (0, 1):
+        for sg in (0, 1):
+            i = site(m, k, s, sg)
+            add(i, i, eps[2 * c + s])
+    for (dm, dk) in ((0, 0), (-1, 0), (0, -1)):
+        d = dAB + dm * a1 + dk * a2
+        dx, dy = d / np.linalg.norm(d)
+        # spin block of the bond A(m,k) -> B: i lr (sx dy - sy dx)
+        blk = 1j * lr * np.array([[0.0, dy + 1j * dx], [dy - 1j * dx, 0.0]])
+        for sa in (0, 1):
+            for sb in (0, 1):
+                iA = site(m, k, 0, sa)
+                jB = site(m + dm, k + dk, 1, sb)
+                v = (-t if sa == sb else 0.0) + blk[sa, sb]
+                if v == 0:
+                    continue
+                add(iA, jB, v)
+                add(jB, iA, np.conj(v))
+    for (dm, dk) in ((1, 0), (-1, 1), (0, -1)):
+        for s, nu in ((0, 1.0), (1, -1.0)):
+            for sg, sz in ((0, 1.0), (1, -1.0)):
+                i = site(m, k, s, sg)
+                j = site(m + dm, k + dk, s, sg)
+                v = 1j * lso * nu * sz
+                add(i, j, v)
+                add(j, i, np.conj(v))
+    return _to_upper_csr(n, np.concatenate(rows), np.concatenate(cols), np.concatenate(vals))
 
 
 def _drop_zeros(m: CSR) -> CSR:

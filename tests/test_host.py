@@ -117,6 +117,18 @@ def test_cubic_lattice():
     assert abs(e.min() + 6.0) < 1e-12 and 2 * m.nnz - m.n == 7 * m.n
 
 
+def test_kane_mele_hermitian_kramers_and_clean_limit():
+    # time reversal (SOC and Rashba both respect it): Kramers pairs at W = 0
+    e = np.sort(np.linalg.eigvalsh(_dense(lattices.kane_mele(6, W=0.0))))
+    assert np.abs(e[0::2] - e[1::2]).max() < 1e-12
+    # lso = lr = 0: two spin copies of graphene, spectrum within [-3, 3], E = +-3 at Gamma
+    g = np.sort(np.linalg.eigvalsh(_dense(lattices.kane_mele(6, lso=0.0, lr=0.0, W=0.0))))
+    assert abs(g[0] + 3.0) < 1e-12 and abs(g[-1] - 3.0) < 1e-12
+    m = lattices.kane_mele(6, W=1.0)
+    assert m.n == 4 * 36 and np.all(m.col >= np.repeat(np.arange(m.n), np.diff(m.rowptr)))
+    assert 2 * m.nnz - m.n == 13 * m.n  # 1 on-site + 3 hops + 3 Rashba flips + 6 second neighbours
+
+
 def _header_symbols():
     txt = open(os.path.join(ROOT, "include", "linqt_amd.h")).read()
     return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(lq_\w+)\(", txt, re.M)))

@@ -13,6 +13,12 @@ formula on 2048 energies.  One step = one full solve of both components
 `--workload dos` runs the DOS configuration instead (square 256 x 256,
 M = 1024, R = 8, Jackson + DOS on 2048 energies; value = M * R / s).
 
+`--workload serve` replays a seeded Poisson stream of DOS, Greenwood and
+Bastin requests (200 req/s; DOS on cubic Anderson / Kane-Mele systems with
+N up to 2.1M and M up to 4096, 2D on Haldane honeycombs) through the serving
+loop (continuous batching, admission by device footprint); one step is the
+whole stream, value = moments served / s, with p50/p99 request latency.
+
 `--workload batch` runs 64 independent DOS instances (cubic Anderson
 L in {16..96}, Kane-Mele-like honeycomb N = 16k..1M, M in {256..2048},
 R in {1..16}, drawn from a fixed seed) through the batched engine; one step
@@ -127,6 +133,24 @@ def batch_instances(lq, ctx, njobs):
     return specs, mats
 
 
+def serve_systems(lq, ctx):
+    """name -> (H~, V_x, V_y, N, W, Ec) for the serving mix."""
+    from linqt_amd import lattices
+
+    out = {}
+    for L in (24, 48, 96):
+        Hh = lattices.cubic_anderson(L, W=2.0, seed=20261207 + L)
+        out[f"cubic{L}"] = (lq.SparseMatrix.from_csr(ctx, Hh).Rescale(1 / 7.01, 0.0), None, None, Hh.n, 14.02, 0.0)
+    for L in (64, 256, 724):
+        Hh = lattices.kane_mele(L, W=1.0, seed=20261207 + L)
+        out[f"kanemele{L}"] = (lq.SparseMatrix.from_csr(ctx, Hh).Rescale(1 / 4.6, 0.0), None, None, Hh.n, 9.2, 0.0)
+    for L in (64, 128, 256):
+        H, Vx, Vy = lattices.haldane_honeycomb(L, seed=20261207 + L)
+        out[f"haldane{L}"] = (lq.SparseMatrix.from_csr(ctx, H).Rescale(1 / 4.6, 0.0), lq.SparseMatrix.from_csr(ctx, Vx),
+                              lq.SparseMatrix.from_csr(ctx, Vy), H.n, 9.2, 0.0)
+    return out
+
+
 def cpu_baseline_batch(specs, mats):
     """Times the CPU restatement on one state of a mid-size instance for 64
     steps and extrapolates by the row-step work (nnz + N R) of the batch."""
@@ -181,7 +205,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("bastin", "dos", "batch"), default="bastin")
+    ap.add_argument("--workload", choices=("bastin", "dos", "batch", "serve"), default="bastin")
+    ap.add_argument("--requests", type=int, default=200, help="requests of the serve workload")
+    ap.add_argument("--rate", type=float, default=200.0, help="mean arrival rate of the serve workload (req/s)")
     ap.add_argument("--jobs", type=int, default=64, help="instances of the batch workload")
     ap.add_argument("--L", type=int, default=None, help="lattice size (cells per side)")
     ap.add_argument("--M", type=int, default=None)
@@ -204,7 +230,7 @@ def main():
     from linqt_amd import lattices
 
     ctx = lq.Context(0)
-    ctx.set_timing(True)
+    ctx.set_timing(False)  # the timed steps run uninstrumented; one extra step is instrumented below
     log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)
 
     if args.calibrate:
@@ -243,6 +269,34 @@ def main():
         config = {"workload": f"kubo_bastin_2d honeycomb-haldane {L}x{L} cells N={n} t2=0.1 W=2 "
                               f"M={M}x{M} R={R} sigma_xx+sigma_xy nump={nump}",
                   "N": n, "M": M, "R": R, "nump": nump, "components": 2, "precision": args.precision}
+    elif args.workload == "serve":
+        from linqt_amd import serve
+
+        systems = serve_systems(lq, ctx)
+        names_1d = [k for k, v in systems.items() if v[1] is None]
+        names_2d = [k for k, v in systems.items() if v[1] is not None]
+        reqs_proto = serve.request_mix(args.requests, args.rate, 20261207, names_1d, names_2d, frac_2d=0.3,
+                                       M_1d=(256, 512, 1024, 2048, 4096), M_2d=(128, 256, 512), R=(1, 2, 4, 8))
+        log(f"serve: {len(reqs_proto)} requests at {args.rate} req/s over {len(systems)} systems")
+        served = []
+
+        def step():
+            reqs = [serve.Request(r.rid, r.kind, r.system, r.M, r.R, r.arrival, r.seed) for r in reqs_proto]
+            be = serve.GpuBackend(lq, ctx, systems, chunk=32)
+            serve.Scheduler(be, hbm_budget=int(64e9)).run(reqs)
+            be.close()
+            served.append(serve.summary(reqs))
+            return reqs
+
+        moments_per_step = sum(r.moments for r in reqs_proto)
+        n = max(v[3] for v in systems.values())
+        M = max(r.M for r in reqs_proto)
+        R = max(r.R for r in reqs_proto)
+        config = {"workload": f"serve: {args.requests} Poisson requests at {args.rate:g} req/s, DOS (cubic Anderson "
+                              f"L24-96, Kane-Mele N16k-2.1M, M256-4096) + Greenwood/Bastin (Haldane N8k-131k, "
+                              f"M128-512), seed 20261207",
+                  "requests": args.requests, "rate": args.rate, "max_N": n, "M": M, "R": R, "precision": "fp64",
+                  "systems": {k: v[3] for k, v in systems.items()}}
     elif args.workload == "batch":
         if args.precision != "fp64":
             sys.exit("bench.py: --precision applies to the 2D (bastin) workload")
@@ -304,19 +358,28 @@ def main():
         log(f"step {k + 1}/{args.steps}: {lat[-1]:.1f} ms")
     ctx.synchronize()
     elapsed = time.perf_counter() - t_start
+    # one more step with HIP events around every launch (on the library's
+    # stream) for the per-kernel figures; it is not part of `value`
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    step()
+    ctx.synchronize()
     tm = ctx.timing()
+    ctx.set_timing(False)
+    tsteps = 1
 
     ms_per_step = elapsed * 1e3 / args.steps
     value = moments_per_step / (elapsed / args.steps)
     spmv_gbs = tm["spmv_bytes"] / (tm["spmv_ms"] * 1e-3) / 1e9 if tm["spmv_ms"] > 0 else 0.0
     ctr_tfs = tm["contract_flops"] / (tm["contract_ms"] * 1e-3) / 1e12 if tm["contract_ms"] > 0 else 0.0
     kernels = {
-        "spmv": {"ms_per_step": tm["spmv_ms"] / args.steps, "launches_per_step": tm["spmv_launches"] / args.steps,
+        "spmv": {"ms_per_step": tm["spmv_ms"] / tsteps, "launches_per_step": tm["spmv_launches"] / tsteps,
                  "GB/s": spmv_gbs, "frac_hbm": spmv_gbs / HBM_PEAK_GBS},
-        "contract": {"ms_per_step": tm["contract_ms"] / args.steps,
-                     "launches_per_step": tm["contract_launches"] / args.steps,
+        "contract": {"ms_per_step": tm["contract_ms"] / tsteps,
+                     "launches_per_step": tm["contract_launches"] / tsteps,
                      "TFLOP/s": ctr_tfs, "frac_fp64_mfma": ctr_tfs / FP64_MFMA_PEAK_TFS},
-        "post_ms_per_step": tm["post_ms"] / args.steps,
+        "post_ms_per_step": tm["post_ms"] / tsteps,
+        "instrumented_step": "one extra step after the timed ones, HIP events around every launch",
     }
     # HBM bytes per launch of the dominant kernel, from the committed PMC summary of this workload
     traffic_tab = {}
@@ -383,6 +446,8 @@ def main():
                 cpu = cpu_baseline_bastin(host_ops, M, R, nump, phi)
             elif args.workload == "batch":
                 cpu = cpu_baseline_batch(specs, mats)
+            elif args.workload == "serve":
+                cpu = None
             else:
                 cpu = cpu_baseline_dos(Hh, M, R, phi)
         except Exception as e:  # the baseline is a report, not the measurement
@@ -411,6 +476,9 @@ def main():
         out["accuracy_vs_fp64"] = accuracy
     if sequential is not None:
         out["sequential_single_job"] = sequential
+    if args.workload == "serve":
+        out["serving"] = served[-1]
+        out["latency_ms"] = served[-1]["latency_ms"]
     print(json.dumps(out), flush=True)
 
 

@@ -19,6 +19,8 @@
 //     (S + 1) 2^{-7S} / 2 relative to 2^(eA + eB).
 //  Re C = Ar Br + Ai Bi and Im C = Ar Bi + Ai (-Br): -Br is formed in
 //  registers with a byte-wise negate, so each level needs two accumulators.
+//  Digit planes are stored step-major (64 K-elements per step), so the rows
+//  a block needs for one step are one contiguous range.
 //
 // Block tile 64 x 64 (S <= 5) or 32 x 64 (S >= 6, register budget), 8 waves of
 // (16 TI) x 16.  A K-step is 64 elements (one MFMA depth); its digits of A and
@@ -40,13 +42,17 @@ constexpr int kOBT = 64;      // block tile
 constexpr int kSliceThreads = 256;
 
 // ---- slicing ----------------------------------------------------------------
-// One block per (row, group).  Q[row][t][c][Kp] int8 (c = 0 real, 1 imag),
-// E[row][ngroups] int32.  Elements k >= K get digit 0.
+// Digit planes, step-major: Q[step][row][plane][64] int8 with step = k / 64,
+// plane = 2 t + c (c = 0 real, 1 imag) and rows padded to a multiple of 64
+// (zero digits), so one K-step of any block's rows is a single contiguous,
+// in-bounds range.  Exponents E[group][padded row] int32.  Elements k >= K get
+// digit 0.  One block per (padded row, group).
 template <int S>
 __global__ __launch_bounds__(kSliceThreads) void k_slice(const double2 *__restrict__ X, long long ldx, long long K,
-                                                         long long Kp, int ngroups, signed char *__restrict__ Q,
+                                                         int rows, int rows_pad, signed char *__restrict__ Q,
                                                          int *__restrict__ E) {
   const int row = blockIdx.y, grp = blockIdx.x;
+  const bool real_row = row < rows;
   const long long g0 = (long long)grp * kGroup;
   constexpr int per = kGroup / kSliceThreads;  // 16 elements per thread
   const long long k0 = g0 + (long long)threadIdx.x * per;
@@ -56,7 +62,7 @@ __global__ __launch_bounds__(kSliceThreads) void k_slice(const double2 *__restri
 #pragma unroll
   for (int j = 0; j < per; ++j) {
     const long long k = k0 + j;
-    v[j] = k < K ? xr[k] : make_double2(0.0, 0.0);
+    v[j] = (real_row && k < K) ? xr[k] : make_double2(0.0, 0.0);
     mx = fmax(mx, fmax(fabs(v[j].x), fabs(v[j].y)));
   }
   __shared__ double red[kSliceThreads];
@@ -71,7 +77,7 @@ __global__ __launch_bounds__(kSliceThreads) void k_slice(const double2 *__restri
     frexp(red[0], &e);  // red[0] < 2^e
     e += 1;             // |x / 2^e| < 1/2: rounded digits stay within [-64, 64]
   }
-  if (threadIdx.x == 0) E[(size_t)row * ngroups + grp] = e;
+  if (threadIdx.x == 0) E[(size_t)grp * rows_pad + row] = e;
   // digits packed four to a register: pr[t][w] holds elements 4w..4w+3
   unsigned pr[S][per / 4], pq[S][per / 4];
 #pragma unroll
@@ -91,11 +97,12 @@ __global__ __launch_bounds__(kSliceThreads) void k_slice(const double2 *__restri
       ai = yi - tq;
     }
   }
-  signed char *qrow = Q + (size_t)row * S * 2 * Kp;
+  const long long step = k0 / kStep;
+  signed char *q = Q + ((size_t)step * rows_pad + row) * (2 * S * kStep) + (k0 % kStep);
 #pragma unroll
   for (int t = 0; t < S; ++t) {
-    *reinterpret_cast<uint4 *>(qrow + ((size_t)t * 2 + 0) * Kp + k0) = make_uint4(pr[t][0], pr[t][1], pr[t][2], pr[t][3]);
-    *reinterpret_cast<uint4 *>(qrow + ((size_t)t * 2 + 1) * Kp + k0) = make_uint4(pq[t][0], pq[t][1], pq[t][2], pq[t][3]);
+    *reinterpret_cast<uint4 *>(q + (2 * t + 0) * kStep) = make_uint4(pr[t][0], pr[t][1], pr[t][2], pr[t][3]);
+    *reinterpret_cast<uint4 *>(q + (2 * t + 1) * kStep) = make_uint4(pq[t][0], pq[t][1], pq[t][2], pq[t][3]);
   }
 }
 
@@ -117,10 +124,11 @@ __device__ __forceinline__ v4i neg(v4i x) { return v4i{neg4(x.x), neg4(x.y), neg
 template <int S, int TI>
 struct I8Layout {
   static constexpr int kBM = 32 * TI;
-  static constexpr int kRowBytes = S * 2 * kStep + 16;  // padded LDS row (bank spread)
+  static constexpr int kRB = 2 * S * kStep;   // bytes of one row in one K-step
+  static constexpr int kRowBytes = kRB + 16;  // padded LDS row (16 rows hit distinct banks)
+  static_assert(kRB % 16 == 0, "rows are whole 16-byte chunks");
   static constexpr int kTileA = kBM * kRowBytes, kTileB = kOBT * kRowBytes;
-  static constexpr int kChunksA = kBM * S * 2 * (kStep / 16);  // 16-byte chunks per operand tile
-  static constexpr int kChunksB = kOBT * S * 2 * (kStep / 16);
+  static constexpr int kChunksA = kBM * kRB / 16, kChunksB = kOBT * kRB / 16;  // 16-byte chunks
   static constexpr int kPerA = (kChunksA + 511) / 512, kPerB = (kChunksB + 511) / 512;
 };
 
@@ -129,15 +137,21 @@ constexpr int tiles_per_wave() {
   return S <= 5 ? 2 : 1;
 }
 
-// grid: tiles_m * tiles_n * nsplit blocks of 512 threads; block b handles K-chunk b % nsplit.
+// grid: tiles_m * tiles_n * nsplit blocks of 512 threads; block b handles
+// K-chunk b % nsplit (whole groups).  Per K-step: barrier, prefetched digits
+// (and, at a group's first step, its exponents) go to LDS, barrier, the next
+// step's loads are issued, then this step's MFMAs run.  The only global
+// accesses inside the loop are those prefetches, so their latency overlaps the
+// MFMAs of a whole step.
 template <int S, int TI>
 __global__ __launch_bounds__(512) void k_contract_i8(const signed char *__restrict__ QA, const int *__restrict__ EA,
                                                      int M0, const signed char *__restrict__ QB,
-                                                     const int *__restrict__ EB, int M1, long long Kp, int ngroups,
+                                                     const int *__restrict__ EB, int M1, int ngroups,
                                                      int groups_per_chunk, int nsplit, int tiles_n,
                                                      double2 *__restrict__ work) {
   using Lay = I8Layout<S, TI>;
   __shared__ __attribute__((aligned(16))) unsigned char lds[Lay::kTileA + Lay::kTileB];
+  __shared__ int sE[2][Lay::kBM + kOBT];  // exponents of the current / next group (by group parity)
   unsigned char *sA = lds, *sB = lds + Lay::kTileA;
   const int bid = blockIdx.x;
   const int chunk = bid % nsplit;
@@ -150,70 +164,63 @@ __global__ __launch_bounds__(512) void k_contract_i8(const signed char *__restri
   const int grp0 = chunk * groups_per_chunk;
   int grp1 = grp0 + groups_per_chunk;
   if (grp1 > ngroups) grp1 = ngroups;
-  const long long kbeg = (long long)grp0 * kGroup;
-  long long kend = (long long)grp1 * kGroup;
-  if (kend > Kp) kend = Kp;
-  const int nsteps = kend > kbeg ? (int)((kend - kbeg) / kStep) : 0;
+  constexpr int steps_per_group = kGroup / kStep;
+  const int st0 = grp0 * steps_per_group;
+  const int nsteps = grp1 > grp0 ? (grp1 - grp0) * steps_per_group : 0;
 
-  // global -> LDS: chunk c of a tile = (row, t, comp, q) with q the 16-byte
-  // quarter of the 64-byte K-step; thread tid moves chunks tid, tid + 512, ...
+  // Chunk c = tid + 512 j of a step's tile: 16 bytes at c * 16 of the
+  // contiguous tile, LDS row c / (kRB / 16) (rows padded by 16 bytes).
+  const int M0p = (M0 + kOBT - 1) / kOBT * kOBT, M1p = (M1 + kOBT - 1) / kOBT * kOBT;
+  const unsigned voff = (unsigned)tid * 16u;
   v4i pa[Lay::kPerA], pb[Lay::kPerB];
-  auto gload = [&](long long kb) {
+  int pe = 0;
+  auto gload = [&](int st) {
+    const signed char *ba = QA + ((size_t)st * M0p + (size_t)tm * Lay::kBM) * Lay::kRB;
+    const signed char *bb = QB + ((size_t)st * M1p + (size_t)tn * kOBT) * Lay::kRB;
 #pragma unroll
-    for (int j = 0; j < Lay::kPerA; ++j) {
-      const int c = tid + j * 512;
-      if (c < Lay::kChunksA) {
-        const int q = c & 3, tc = (c >> 2) % (S * 2), row = c / (S * 2 * 4);
-        int ra = tm * Lay::kBM + row;
-        if (ra >= M0) ra = M0 - 1;
-        pa[j] = *reinterpret_cast<const v4i *>(QA + ((size_t)ra * S * 2 + tc) * Kp + kb + q * 16);
-      }
-    }
+    for (int j = 0; j < Lay::kPerA; ++j)
+      if (tid + j * 512 < Lay::kChunksA) pa[j] = *reinterpret_cast<const v4i *>(ba + (size_t)j * 8192 + voff);
 #pragma unroll
-    for (int j = 0; j < Lay::kPerB; ++j) {
-      const int c = tid + j * 512;
-      if (c < Lay::kChunksB) {
-        const int q = c & 3, tc = (c >> 2) % (S * 2), row = c / (S * 2 * 4);
-        int rb = tn * kOBT + row;
-        if (rb >= M1) rb = M1 - 1;
-        pb[j] = *reinterpret_cast<const v4i *>(QB + ((size_t)rb * S * 2 + tc) * Kp + kb + q * 16);
-      }
+    for (int j = 0; j < Lay::kPerB; ++j)
+      if (tid + j * 512 < Lay::kChunksB) pb[j] = *reinterpret_cast<const v4i *>(bb + (size_t)j * 8192 + voff);
+    if (st % steps_per_group == 0) {
+      const int g = st / steps_per_group;
+      if (tid < Lay::kBM) pe = EA[(size_t)g * M0p + tm * Lay::kBM + tid];
+      else if (tid < Lay::kBM + kOBT) pe = EB[(size_t)g * M1p + tn * kOBT + tid - Lay::kBM];
     }
   };
-  auto lstore = [&]() {
+  auto lstore = [&](int st) {
 #pragma unroll
     for (int j = 0; j < Lay::kPerA; ++j) {
       const int c = tid + j * 512;
-      if (c < Lay::kChunksA) {
-        const int q = c & 3, tc = (c >> 2) % (S * 2), row = c / (S * 2 * 4);
-        *reinterpret_cast<v4i *>(sA + row * Lay::kRowBytes + tc * kStep + q * 16) = pa[j];
-      }
+      if (c < Lay::kChunksA) *reinterpret_cast<v4i *>(sA + 16 * (c + c / (Lay::kRB / 16))) = pa[j];
     }
 #pragma unroll
     for (int j = 0; j < Lay::kPerB; ++j) {
       const int c = tid + j * 512;
-      if (c < Lay::kChunksB) {
-        const int q = c & 3, tc = (c >> 2) % (S * 2), row = c / (S * 2 * 4);
-        *reinterpret_cast<v4i *>(sB + row * Lay::kRowBytes + tc * kStep + q * 16) = pb[j];
-      }
+      if (c < Lay::kChunksB) *reinterpret_cast<v4i *>(sB + 16 * (c + c / (Lay::kRB / 16))) = pb[j];
     }
+    if (st % steps_per_group == 0 && tid < Lay::kBM + kOBT) sE[(st / steps_per_group) & 1][tid] = pe;
   };
 
   v4i accR[TI][S], accI[TI][S];
+  double fR[TI][4], fI[TI][4];
 #pragma unroll
-  for (int ti = 0; ti < TI; ++ti)
+  for (int ti = 0; ti < TI; ++ti) {
 #pragma unroll
     for (int l = 0; l < S; ++l) accR[ti][l] = accI[ti][l] = v4i{0, 0, 0, 0};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) fR[ti][r] = fI[ti][r] = 0.0;
+  }
   // output element r of lane l in a 16x16 tile: row 4*(l/16) + r, column l % 16
-  const int col = tn * kOBT + wn + r16;
 
-  const int steps_per_group = kGroup / kStep;
-  if (nsteps > 0) gload(kbeg);
-  for (int st = 0; st < nsteps; ++st) {
+  if (nsteps > 0) gload(st0);
+  for (int i = 0; i < nsteps; ++i) {
+    const int st = st0 + i;
     __syncthreads();  // previous step's LDS reads are done
-    lstore();
+    lstore(st);
     __syncthreads();
-    if (st + 1 < nsteps) gload(kbeg + (long long)(st + 1) * kStep);
+    if (i + 1 < nsteps) gload(st + 1);
     // digits of B for this wave's column tile (all slices), -Br formed here
     v4i br[S], bi[S], nbr[S];
 #pragma unroll
@@ -233,37 +240,29 @@ __global__ __launch_bounds__(512) void k_contract_i8(const signed char *__restri
 #pragma unroll
         for (int u = 0; u + t < S; ++u) {
           accR[ti][t + u] = mfma_i8(ar, br[u], accR[ti][t + u]);
-          accR[ti][t + u] = mfma_i8(ai, bi[u], accR[ti][t + u]);
           accI[ti][t + u] = mfma_i8(ar, bi[u], accI[ti][t + u]);
+          accR[ti][t + u] = mfma_i8(ai, bi[u], accR[ti][t + u]);
           accI[ti][t + u] = mfma_i8(ai, nbr[u], accI[ti][t + u]);
         }
       }
     }
-    // end of an exponent group: fold the exact level sums, in FP64, into this
-    // block's own partial table (no other block touches it)
-    if ((st + 1) % steps_per_group == 0 || st + 1 == nsteps) {
-      const int grp = (int)((kbeg + (long long)st * kStep) / kGroup);
-      const bool first = grp == grp0;
-      const int eb = col < M1 ? EB[(size_t)col * ngroups + grp] : 0;
+    // end of an exponent group: fold the exact level sums into FP64
+    if ((i + 1) % steps_per_group == 0) {
+      const int* se = sE[(st / steps_per_group) & 1];
+      const int eb = se[Lay::kBM + wn + r16];
 #pragma unroll
       for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = tm * Lay::kBM + wm + ti * 16 + kq * 4 + r;
-          if (row < M0 && col < M1) {
-            const int ea = EA[(size_t)row * ngroups + grp];
-            double sr = 0.0, si = 0.0;
+          const int ea = se[wm + ti * 16 + kq * 4 + r];
+          double sr = 0.0, si = 0.0;
 #pragma unroll
-            for (int l = S - 1; l >= 0; --l) {  // smallest weights first
-              sr += ldexp((double)accR[ti][l][r], -7 * (l + 2));
-              si += ldexp((double)accI[ti][l][r], -7 * (l + 2));
-            }
-            double2 *w = work + ((size_t)chunk * M0 + row) * M1 + col;
-            double2 acc = first ? make_double2(0.0, 0.0) : *w;
-            acc.x += ldexp(sr, ea + eb);
-            acc.y += ldexp(si, ea + eb);
-            *w = acc;
+          for (int l = S - 1; l >= 0; --l) {  // smallest weights first
+            sr += ldexp((double)accR[ti][l][r], -7 * (l + 2));
+            si += ldexp((double)accI[ti][l][r], -7 * (l + 2));
           }
+          fR[ti][r] += ldexp(sr, ea + eb);
+          fI[ti][r] += ldexp(si, ea + eb);
         }
 #pragma unroll
       for (int ti = 0; ti < TI; ++ti)
@@ -271,26 +270,27 @@ __global__ __launch_bounds__(512) void k_contract_i8(const signed char *__restri
         for (int l = 0; l < S; ++l) accR[ti][l] = accI[ti][l] = v4i{0, 0, 0, 0};
     }
   }
-  if (nsteps == 0) {  // empty K-chunk: its partial table is zero
+  const int col = tn * kOBT + wn + r16;
 #pragma unroll
-    for (int ti = 0; ti < TI; ++ti)
+  for (int ti = 0; ti < TI; ++ti)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = tm * Lay::kBM + wm + ti * 16 + kq * 4 + r;
-        if (row < M0 && col < M1) work[((size_t)chunk * M0 + row) * M1 + col] = make_double2(0.0, 0.0);
-      }
-  }
+    for (int r = 0; r < 4; ++r) {
+      const int row = tm * Lay::kBM + wm + ti * 16 + kq * 4 + r;
+      if (row < M0 && col < M1) work[((size_t)chunk * M0 + row) * M1 + col] = make_double2(fR[ti][r], fI[ti][r]);
+    }
 }
 
 }  // namespace
 
 long long ozaki_kp(long long K) { return ((K + kGroup - 1) / kGroup) * kGroup; }
 
+int pad_rows(int rows) { return (rows + kOBT - 1) / kOBT * kOBT; }
+
 int ozaki_slice_bytes(int rows, long long K, int S, size_t *qbytes, size_t *ebytes) {
   const long long Kp = ozaki_kp(K);
   const long long ng = Kp / kGroup;
-  if (qbytes) *qbytes = (size_t)rows * S * 2 * Kp;
-  if (ebytes) *ebytes = (size_t)rows * ng * sizeof(int);
+  if (qbytes) *qbytes = (size_t)pad_rows(rows) * S * 2 * Kp;
+  if (ebytes) *ebytes = (size_t)pad_rows(rows) * ng * sizeof(int);
   return LQ_OK;
 }
 
@@ -298,11 +298,11 @@ int launch_slice(hipStream_t st, const double2 *X, long long ldx, int rows, long
                  int *E) {
   const long long Kp = ozaki_kp(K);
   const int ng = (int)(Kp / kGroup);
-  dim3 grid(ng, rows);
+  dim3 grid(ng, pad_rows(rows));
   switch (S) {
 #define LQ_SL(SV)                                                                                         \
   case SV:                                                                                               \
-    hipLaunchKernelGGL((k_slice<SV>), grid, dim3(kSliceThreads), 0, st, X, ldx, K, Kp, ng, Q, E); \
+    hipLaunchKernelGGL((k_slice<SV>), grid, dim3(kSliceThreads), 0, st, X, ldx, K, rows, pad_rows(rows), Q, E); \
     break;
     LQ_SL(4)
     LQ_SL(5)
@@ -339,7 +339,7 @@ int launch_contract_i8(hipStream_t st, const signed char *QA, const int *EA, int
     using Lay = I8Layout<SV, tiles_per_wave<SV>()>;                                                      \
     const long long blocks = (long long)((M0 + Lay::kBM - 1) / Lay::kBM) * tiles_n * nsplit;            \
     hipLaunchKernelGGL((k_contract_i8<SV, tiles_per_wave<SV>()>), dim3((unsigned)blocks), dim3(512), 0, st, QA, \
-                       EA, M0, QB, EB, M1, Kp, ng, gpc, nsplit, tiles_n, work);                                 \
+                       EA, M0, QB, EB, M1, ng, gpc, nsplit, tiles_n, work);                                     \
     break;                                                                                              \
   }
     LQ_CI(4)

@@ -1,0 +1,211 @@
+"""Serving loop for moment requests on one GPU (continuous batching).
+
+Requests arrive over time (a seeded Poisson stream in the benchmark):
+
+* ``dos``: SpectralMoments (chebyshev_solver.cpp:176-200) + Jackson + DOS on
+  an energy grid -- these join the batched engine (``Batch``) at the next
+  chunk boundary, so many of them advance together;
+* ``greenwood`` / ``bastin``: Kubo_solver_FFT::compute (Kubo_solver_FFT.cpp)
+  + post-processing -- run as whole solves between chunks, oldest first.
+
+Admission is by device footprint: a request starts only when the bytes it
+holds while running fit the server's HBM budget next to what is already
+running; otherwise it waits in arrival order (first fit, so a small request
+may pass a large one).  Every request's result is what the same request
+gives when served alone: the batched engine's moments are bitwise independent
+of the mix, and a 2D solve runs with the same inputs either way.
+
+The scheduler (``Scheduler``) only sees a backend with ``footprint``,
+``start_1d``, ``step_1d``, ``run_2d`` and ``collect_1d``; ``GpuBackend``
+implements them on the C ABI.
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+from typing import Callable
+
+import numpy as np
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    kind: str              # "dos", "greenwood", "bastin"
+    system: str            # key of the operator set in the backend's registry
+    M: int
+    R: int
+    arrival: float         # seconds from the start of the run
+    seed: int = 0
+    nump: int = 0          # energies (0: 2 M for 2D, 2048 for DOS)
+    start: float | None = None
+    finish: float | None = None
+    result: object = None
+
+    @property
+    def moments(self) -> int:
+        return self.M * self.R if self.kind == "dos" else self.M * self.M * self.R
+
+    @property
+    def latency(self) -> float | None:
+        return None if self.finish is None else self.finish - self.arrival
+
+
+def poisson_arrivals(n: int, rate: float, seed: int) -> np.ndarray:
+    """Arrival times of n requests with exponential gaps of mean 1/rate."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))
+
+
+def request_mix(n: int, rate: float, seed: int, systems_1d: list[str], systems_2d: list[str],
+                frac_2d: float = 0.3, M_1d=(256, 512, 1024, 2048, 4096), M_2d=(128, 256), R=(1, 2, 4, 8)):
+    """n seeded requests: a fraction frac_2d Greenwood/Bastin, the rest DOS."""
+    rng = np.random.default_rng(seed + 1)
+    t = poisson_arrivals(n, rate, seed)
+    out = []
+    for i in range(n):
+        if systems_2d and rng.uniform() < frac_2d:
+            kind = "greenwood" if rng.uniform() < 0.5 else "bastin"
+            out.append(Request(i, kind, str(rng.choice(systems_2d)), int(rng.choice(M_2d)),
+                               int(rng.choice(R[:2])), float(t[i]), seed=seed * 1000 + i))
+        else:
+            out.append(Request(i, "dos", str(rng.choice(systems_1d)), int(rng.choice(M_1d)),
+                               int(rng.choice(R)), float(t[i]), seed=seed * 1000 + i))
+    return out
+
+
+class Scheduler:
+    """Arrival-ordered, footprint-admitted continuous batching."""
+
+    def __init__(self, backend, hbm_budget: int, clock: Callable[[], float] | None = None,
+                 sleep: Callable[[float], None] | None = None):
+        self.b = backend
+        self.budget = hbm_budget
+        self.clock = clock or time.perf_counter
+        self.sleep = sleep or time.sleep
+        self.used = 0
+        self.log: list[tuple[float, str, int]] = []
+
+    def run(self, requests: list[Request]) -> list[Request]:
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        for r in pending:
+            if self.b.footprint(r) > self.budget:
+                raise ValueError(f"request {r.rid} needs {self.b.footprint(r)} bytes, above the budget")
+        waiting: list[Request] = []
+        running_1d: dict[int, Request] = {}
+        t0 = self.clock()
+        now = lambda: self.clock() - t0
+        while pending or waiting or running_1d:
+            while pending and pending[0].arrival <= now():
+                waiting.append(pending.pop(0))
+            # first-fit admission of 1D requests (they share the batched engine)
+            for r in list(waiting):
+                if r.kind == "dos" and self.used + self.b.footprint(r) <= self.budget:
+                    waiting.remove(r)
+                    self.used += self.b.footprint(r)
+                    r.start = now()
+                    self.b.start_1d(r)
+                    running_1d[r.rid] = r
+                    self.log.append((r.start, "start", r.rid))
+            # the oldest 2D request that fits runs whole, between chunks
+            for r in list(waiting):
+                if r.kind != "dos" and self.used + self.b.footprint(r) <= self.budget:
+                    waiting.remove(r)
+                    r.start = now()
+                    self.log.append((r.start, "start", r.rid))
+                    r.result = self.b.run_2d(r)
+                    r.finish = now()
+                    self.log.append((r.finish, "finish", r.rid))
+                    break
+            if running_1d:
+                for rid in self.b.step_1d():
+                    r = running_1d.pop(rid)
+                    r.result = self.b.collect_1d(r)
+                    r.finish = now()
+                    self.used -= self.b.footprint(r)
+                    self.log.append((r.finish, "finish", r.rid))
+            elif not waiting and pending:
+                self.sleep(max(0.0, pending[0].arrival - now()))
+        return requests
+
+
+def summary(requests: list[Request]) -> dict:
+    done = [r for r in requests if r.finish is not None]
+    lat = np.array([r.latency for r in done]) * 1e3
+    span = max(r.finish for r in done) - min(r.arrival for r in done)
+    out = {"requests": len(done), "makespan_s": span, "throughput_req_s": len(done) / span,
+           "moments_per_s": sum(r.moments for r in done) / span,
+           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                          "max": float(lat.max())}}
+    for kind in ("dos", "greenwood", "bastin"):
+        lk = [r.latency * 1e3 for r in done if r.kind == kind]
+        if lk:
+            out[f"latency_ms_{kind}"] = {"n": len(lk), "p50": float(np.percentile(lk, 50)),
+                                        "p99": float(np.percentile(lk, 99))}
+    return out
+
+
+class GpuBackend:
+    """Runs requests on one context.  ``systems`` maps a name to
+    (H, V_x, V_y, n, W, Ec) with H already rescaled (V may be None for DOS-only)."""
+
+    def __init__(self, lq, ctx, systems: dict, chunk: int = 32, hbm_budget: int = 0):
+        self.lq = lq
+        self.ctx = ctx
+        self.systems = systems
+        self.batch = lq.Batch(ctx, hbm_budget=hbm_budget, chunk=chunk)
+        self.chunk = chunk
+        self.jobs: dict[int, int] = {}
+
+    def states(self, r: Request):
+        from .lattices import random_phase_states
+
+        return random_phase_states(self.systems[r.system][3], r.R, seed=r.seed)
+
+    def footprint(self, r: Request) -> int:
+        n = self.systems[r.system][3]
+        if r.kind == "dos":
+            return self.lq.Batch.job_bytes(n, r.R, r.M, self.chunk)
+        # stored L and R vectors of both sides (one group of R states) + work vectors
+        return (2 * r.M + 4) * n * r.R * 16
+
+    def start_1d(self, r: Request):
+        self.jobs[self.batch.submit(self.systems[r.system][0], self.states(r), r.M)] = r.rid
+
+    def step_1d(self) -> list[int]:
+        self.batch.run(max_chunks=1)
+        done = [j for j in self.jobs if self.batch.state(j) == 2]
+        self._done = {self.jobs[j]: j for j in done}
+        return [self.jobs.pop(j) for j in done]
+
+    def collect_1d(self, r: Request):
+        lq = self.lq
+        H, _, _, n, W, Ec = self.systems[r.system]
+        mu = self.batch.result(self._done[r.rid])
+        muj = lq.jackson_1d(self.ctx, mu, W, 1e-3)
+        return mu, lq.dos_fftgrid(self.ctx, muj, n, W, Ec, r.nump or 2048)
+
+    def run_2d(self, r: Request):
+        lq = self.lq
+        H, Vx, Vy, n, W, Ec = self.systems[r.system]
+        formula = lq.LQ_KUBO_GREENWOOD if r.kind == "greenwood" else lq.LQ_KUBO_BASTIN
+        nump = r.nump or 2 * r.M
+        fd = lq.kubo_fft(H, Vx, Vx, self.states(r), r.M, nump, formula)
+        return fd, lq.kubo_post(self.ctx, formula, fd, n, W, Ec)
+
+    def alone(self, r: Request):
+        """The same request served by itself (fresh batch for DOS)."""
+        if r.kind != "dos":
+            return self.run_2d(r)
+        lq = self.lq
+        b = lq.Batch(self.ctx, chunk=self.chunk)
+        j = b.submit(self.systems[r.system][0], self.states(r), r.M)
+        b.run()
+        mu = b.result(j)
+        b.close()
+        H, _, _, n, W, Ec = self.systems[r.system]
+        muj = lq.jackson_1d(self.ctx, mu, W, 1e-3)
+        return mu, lq.dos_fftgrid(self.ctx, muj, n, W, Ec, r.nump or 2048)
+
+    def close(self):
+        self.batch.close()

@@ -7,6 +7,8 @@
 #include <cstring>
 #include <numeric>
 
+#include <map>
+
 #include "lq_internal.h"
 
 namespace lq {
@@ -117,7 +119,103 @@ void free_dev(SellDev &d) {
   if (d.sptr) (void)hipFree(d.sptr);
   if (d.col) (void)hipFree(d.col);
   if (d.val) (void)hipFree(d.val);
+  if (d.dsptr) (void)hipFree(d.dsptr);
+  if (d.wcol) (void)hipFree(d.wcol);
+  if (d.dict) (void)hipFree(d.dict);
+  if (d.diag) (void)hipFree(d.diag);
   d = SellDev{};
+}
+
+// Dictionary form of the completed CSR (see SellDev), when the operator has
+// n < 2^24 rows and at most 256 distinct off-diagonal values (a zero value
+// counts when padding is needed).  Leaves d.ndict = 0 otherwise.
+int build_dict_form(const lq_csr A, SellDev &d) {
+  const int n = A->n;
+  if (n >= (1 << 24)) return LQ_OK;
+  const char *env = getenv("LQ_SPMM_DICT");
+  if (env && env[0] == '0') return LQ_OK;
+  const int nslices = (n + kSlice - 1) / kSlice;
+  std::map<std::pair<double, double>, int> index;
+  std::vector<double2> dict;
+  std::vector<double2> diag(n, d2(0.0, 0.0));
+  bool has_diag = false;
+  std::vector<int> offd(n, 0);
+  for (int i = 0; i < n; ++i)
+    for (int q = A->rowptr[i]; q < A->rowptr[i + 1]; ++q) {
+      if (A->col[q] == i) {
+        diag[i].x += A->val[q].real();
+        diag[i].y += A->val[q].imag();
+        has_diag = true;
+        continue;
+      }
+      ++offd[i];
+      const auto key = std::make_pair(A->val[q].real(), A->val[q].imag());
+      if (!index.count(key)) {
+        if (dict.size() == 256) return LQ_OK;
+        index.emplace(key, (int)dict.size());
+        dict.push_back(d2(key.first, key.second));
+      }
+    }
+  std::vector<int> sptr(nslices + 1, 0);
+  int wmax = 0;
+  bool padded = false;
+  for (int s = 0; s < nslices; ++s) {
+    int w = 0, wmin = 1 << 30;
+    for (int r = 0; r < kSlice; ++r) {
+      const int i = s * kSlice + r;
+      const int len = i < n ? offd[i] : 0;
+      w = std::max(w, len);
+      wmin = std::min(wmin, len);
+    }
+    if (wmin < w) padded = true;
+    wmax = std::max(wmax, w);
+    const long long next = (long long)sptr[s] + (long long)w * kSlice;
+    if (next > 0x7fffffffLL) return LQ_OK;
+    sptr[s + 1] = (int)next;
+  }
+  int zero = -1;
+  if (padded) {
+    const auto key = std::make_pair(0.0, 0.0);
+    if (!index.count(key)) {
+      if (dict.size() == 256) return LQ_OK;
+      index.emplace(key, (int)dict.size());
+      dict.push_back(d2(0.0, 0.0));
+    }
+    zero = index[key];
+  }
+  const size_t stored = (size_t)sptr[nslices];
+  std::vector<unsigned> wcol(stored > 0 ? stored : 1, 0u);
+  for (int s = 0; s < nslices; ++s) {
+    const int w = (sptr[s + 1] - sptr[s]) / kSlice;
+    for (int r = 0; r < kSlice; ++r) {
+      const int i = s * kSlice + r;
+      int j = 0;
+      if (i < n)
+        for (int q = A->rowptr[i]; q < A->rowptr[i + 1]; ++q) {
+          if (A->col[q] == i) continue;
+          const int id = index[std::make_pair(A->val[q].real(), A->val[q].imag())];
+          wcol[(size_t)sptr[s] + (size_t)j * kSlice + r] = (unsigned)A->col[q] | ((unsigned)id << 24);
+          ++j;
+        }
+      for (; j < w; ++j)
+        wcol[(size_t)sptr[s] + (size_t)j * kSlice + r] = (unsigned)(i < n ? i : 0) | ((unsigned)zero << 24);
+    }
+  }
+  d.ndict = (int)std::max<size_t>(dict.size(), 1);
+  if (dict.empty()) dict.push_back(d2(0.0, 0.0));
+  d.dstored = (long long)stored;
+  d.dwmax = wmax;
+  LQ_HIP_CHECK(hipMalloc(&d.dsptr, sizeof(int) * sptr.size()));
+  LQ_HIP_CHECK(hipMalloc(&d.wcol, sizeof(unsigned) * wcol.size()));
+  LQ_HIP_CHECK(hipMalloc(&d.dict, sizeof(double2) * dict.size()));
+  LQ_HIP_CHECK(hipMemcpy(d.dsptr, sptr.data(), sizeof(int) * sptr.size(), hipMemcpyHostToDevice));
+  LQ_HIP_CHECK(hipMemcpy(d.wcol, wcol.data(), sizeof(unsigned) * wcol.size(), hipMemcpyHostToDevice));
+  LQ_HIP_CHECK(hipMemcpy(d.dict, dict.data(), sizeof(double2) * dict.size(), hipMemcpyHostToDevice));
+  if (has_diag) {
+    LQ_HIP_CHECK(hipMalloc(&d.diag, sizeof(double2) * n));
+    LQ_HIP_CHECK(hipMemcpy(d.diag, diag.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+  }
+  return LQ_OK;
 }
 
 // Lays the completed CSR out as sliced ELL (64-row slices, column-major
@@ -174,8 +272,9 @@ int upload_sell(lq_csr A) {
   LQ_HIP_CHECK(hipMemcpy(d.sptr, sptr.data(), sizeof(int) * sptr.size(), hipMemcpyHostToDevice));
   LQ_HIP_CHECK(hipMemcpy(d.col, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
   LQ_HIP_CHECK(hipMemcpy(d.val, val.data(), sizeof(double2) * val.size(), hipMemcpyHostToDevice));
+  int rc = build_dict_form(A, d);
   A->dev = d;
-  return LQ_OK;
+  return rc;
 }
 
 struct Entry {

@@ -97,6 +97,11 @@ int admit(lq_batch b, Job &j) {
   d.sptr = j.H->dev.sptr;
   d.col = j.H->dev.col;
   d.val = j.H->dev.val;
+  d.dsptr = j.H->dev.dsptr;
+  d.wcol = j.H->dev.wcol;
+  d.dict = j.H->dev.dict;
+  d.diag = j.H->dev.diag;
+  d.ndict = j.H->dev.ndict;
   d.phi = (const double2 *)p;
   d.buf0 = (double2 *)(p + vec);
   d.buf1 = (double2 *)(p + 2 * vec);

@@ -40,6 +40,12 @@ enum TimeCat { kCatSpmv = 0, kCatContract = 1, kCatPost = 2, kNumCats = 3 };
 // Device sliced-ELL operator.  Entry j of row r of slice s lives at
 // sptr[s] + j*kSlice + (r % kSlice); padding entries have col = own row and
 // value 0.
+//
+// Dictionary form (lattice operators: few distinct off-diagonal values): the
+// diagonal is a separate [n] array and only off-diagonal entries are stored,
+// each as one 32-bit word col | (value index << 24) into a table of at most
+// 256 values (padding: own row and a zero value).  4 bytes per stored entry
+// instead of 20; built whenever the operator qualifies, next to the plain form.
 struct SellDev {
   int n = 0;
   int nslices = 0;
@@ -48,6 +54,14 @@ struct SellDev {
   double2 *val = nullptr;// [stored]
   long long stored = 0;
   int wmax = 0;          // widest slice (entries per row)
+  // dictionary form (ndict == 0: absent)
+  int ndict = 0;
+  int *dsptr = nullptr;      // [nslices+1]
+  unsigned *wcol = nullptr;  // [dstored]
+  double2 *dict = nullptr;   // [ndict]
+  double2 *diag = nullptr;   // [n] (nullptr: zero diagonal)
+  long long dstored = 0;
+  int dwmax = 0;
 };
 
 // Arguments of one vector set of a (multi-set) SpMM launch: y = a*A*x + b*z
@@ -72,6 +86,12 @@ struct BatchJobDev {
   const int *sptr;
   const int *col;
   const double2 *val;
+  // dictionary form of the operator (ndict == 0: use sptr/col/val)
+  const int *dsptr;
+  const unsigned *wcol;
+  const double2 *dict;
+  const double2 *diag;
+  int ndict;
   const double2 *phi;  // [n][k] states
   double2 *buf0, *buf1;  // T_s ring
   double2 *partial;      // [chunk slot][nblk][k-width] dot partials

@@ -64,13 +64,44 @@ constexpr int kXcds = 8;
 
 __device__ __forceinline__ int xcd_group(int b, int q) { return (b % kXcds) * q + b / kXcds; }
 
+// Entry accessors of the two operator forms, in two phases so that all
+// loads of a chunk's stored words are in flight before any value that depends
+// on them is fetched: load(p, w, a) reads stored slot p (plain: column and
+// value; dictionary: the packed word), decode(w, c, a) gives the column and
+// value (dictionary: looks the value up).  add_diag adds a separate diagonal.
+struct PlainMat {
+  const int *__restrict__ sptr;
+  const int *__restrict__ col;
+  const double2 *__restrict__ val;
+  __device__ __forceinline__ void load(size_t p, unsigned &w, double2 &a) const {
+    w = (unsigned)col[p];
+    a = val[p];
+  }
+  __device__ __forceinline__ void decode(unsigned w, int &c, double2 &) const { c = (int)w; }
+  __device__ __forceinline__ bool has_diag() const { return false; }
+  __device__ __forceinline__ double2 diag(int) const { return make_double2(0.0, 0.0); }
+};
+
+struct DictMat {
+  const int *__restrict__ sptr;
+  const unsigned *__restrict__ wcol;
+  const double2 *__restrict__ dict;  // <= 256 entries: stays in the vector L1 cache
+  const double2 *__restrict__ dg;
+  __device__ __forceinline__ void load(size_t p, unsigned &w, double2 &) const { w = wcol[p]; }
+  __device__ __forceinline__ void decode(unsigned w, int &c, double2 &a) const {
+    c = (int)(w & 0xffffffu);
+    a = dict[w >> 24];
+  }
+  __device__ __forceinline__ bool has_diag() const { return dg != nullptr; }
+  __device__ __forceinline__ double2 diag(int row) const { return dg[row]; }
+};
+
 // Rows of row groups g0, g0 + gstep, ... (RPB = 256 / KP rows each) of one
 // launch; dot contributions are accumulated into d.
-template <int KP, int DOT, int kUnroll>
-__device__ __forceinline__ void spmm_body(int n, const int *__restrict__ sptr, const int *__restrict__ col,
-                                          const double2 *__restrict__ val, double2 alpha, double2 beta,
-                                          const SetArgs &sa, int nrowgroups, const double2 *__restrict__ phi,
-                                          int g0, int gstep, double2 (&d)[2]) {
+template <int KP, int DOT, int kUnroll, class Mat>
+__device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, double2 beta, const SetArgs &sa,
+                                          int nrowgroups, const double2 *__restrict__ phi, int g0, int gstep,
+                                          double2 (&d)[2]) {
   constexpr int RPB = 256 / KP;
   const int tid = threadIdx.x;
   const int v = tid % KP;
@@ -85,26 +116,41 @@ __device__ __forceinline__ void spmm_body(int n, const int *__restrict__ sptr, c
     const int row = g * RPB + tid / KP;
     if (row < n && lane_on) {
       const int s = row >> 6, r = row & 63;
-      const int base = sptr[s];
-      const int w = (sptr[s + 1] - base) >> 6;
-      const int *__restrict__ cp = col + base + r;
-      const double2 *__restrict__ vp = val + base + r;
+      const int base = mat.sptr[s];
+      const int w = (mat.sptr[s + 1] - base) >> 6;
+      const size_t p0 = (size_t)base + r;
+      const size_t o = (size_t)row * kk + vv;
+      // the separate diagonal (dictionary form) is loaded with the first
+      // entries so it adds no memory round trip of its own
+      double2 dv = make_double2(0.0, 0.0), xo = make_double2(0.0, 0.0);
+      if (mat.has_diag()) {
+        dv = mat.diag(row);
+        xo = x[o];
+      }
       double ar = 0.0, ai = 0.0;
       // Chunks of kUnroll entries: all column/value loads of a chunk are in
       // flight together, then all of its gathers, so a lattice row costs two
       // memory latencies instead of one per entry.
       for (int j0 = 0; j0 < w; j0 += kUnroll) {
+        unsigned wd[kUnroll];
         int c[kUnroll];
         double2 a[kUnroll];
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-          const bool on = j0 + u < w;
-          c[u] = on ? cp[(j0 + u) * kSlice] : row;
-          a[u] = on ? vp[(j0 + u) * kSlice] : make_double2(0.0, 0.0);
+          wd[u] = 0u;
+          a[u] = make_double2(0.0, 0.0);
+          if (j0 + u < w) mat.load(p0 + (size_t)(j0 + u) * kSlice, wd[u], a[u]);
         }
         double2 xv[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) xv[u] = (j0 + u < w) ? x[(size_t)c[u] * kk + vv] : make_double2(0.0, 0.0);
+        for (int u = 0; u < kUnroll; ++u) {
+          if (j0 + u < w) {
+            mat.decode(wd[u], c[u], a[u]);
+            xv[u] = x[(size_t)c[u] * kk + vv];
+          } else {
+            xv[u] = make_double2(0.0, 0.0);
+          }
+        }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
           ar = fma(a[u].x, xv[u].x, ar);
@@ -113,7 +159,12 @@ __device__ __forceinline__ void spmm_body(int n, const int *__restrict__ sptr, c
           ai = fma(a[u].y, xv[u].x, ai);
         }
       }
-      const size_t o = (size_t)row * kk + vv;
+      if (mat.has_diag()) {
+        ar = fma(dv.x, xo.x, ar);
+        ar = fma(-dv.y, xo.y, ar);
+        ai = fma(dv.x, xo.y, ai);
+        ai = fma(dv.y, xo.x, ai);
+      }
       double2 res = make_double2(alpha.x * ar - alpha.y * ai, alpha.x * ai + alpha.y * ar);
       if (z) {
         const double2 zv = z[o];
@@ -138,17 +189,33 @@ __device__ __forceinline__ void spmm_body(int n, const int *__restrict__ sptr, c
   }
 }
 
-template <int KP, int DOT, int kUnroll>
-__global__ __launch_bounds__(256) void k_spmm(int n, const int *__restrict__ sptr, const int *__restrict__ col,
-                                              const double2 *__restrict__ val, double2 alpha, double2 beta,
-                                              SetArgs sa, int nrowgroups, const double2 *__restrict__ phi,
+// The operator as seen by one launch: plain or dictionary form.
+struct MatArgs {
+  const int *sptr;
+  const int *col;
+  const double2 *val;
+  const unsigned *wcol;
+  const double2 *dict;
+  const double2 *diag;
+  int ndict;  // > 0: dictionary form
+};
+
+template <int KP, int DOT, int kUnroll, bool DICT>
+__global__ __launch_bounds__(256) void k_spmm(int n, MatArgs m, double2 alpha, double2 beta, SetArgs sa,
+                                              int nrowgroups, const double2 *__restrict__ phi,
                                               double2 *__restrict__ partial, int q) {
   constexpr int NDOT = (DOT == kDotDoubling) ? 2 : 1;
   double2 d[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
   // q > 0: one row group per block, XCD-remapped; q == 0: grid-stride loop
   const int g0 = q > 0 ? xcd_group(blockIdx.x, q) : blockIdx.x;
   const int gstep = q > 0 ? nrowgroups : gridDim.x;
-  spmm_body<KP, DOT, kUnroll>(n, sptr, col, val, alpha, beta, sa, nrowgroups, phi, g0, gstep, d);
+  if (DICT) {
+    const DictMat mat{m.sptr, m.wcol, m.dict, m.diag};
+    spmm_body<KP, DOT, kUnroll>(n, mat, alpha, beta, sa, nrowgroups, phi, g0, gstep, d);
+  } else {
+    const PlainMat mat{m.sptr, m.col, m.val};
+    spmm_body<KP, DOT, kUnroll>(n, mat, alpha, beta, sa, nrowgroups, phi, g0, gstep, d);
+  }
   if (DOT != kDotNone) block_reduce_store<KP, NDOT>(d, partial, blockIdx.x);
 }
 
@@ -185,7 +252,13 @@ __global__ __launch_bounds__(256) void k_spmm_batch(const BatchJobDev *__restric
   constexpr int RPB = 256 / KP;
   const int groups = (J.n + RPB - 1) / RPB;
   double2 d[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
-  spmm_body<KP, kDotPhi, kUnroll>(J.n, J.sptr, J.col, J.val, alpha, beta, sa, groups, J.phi, lb, J.nblk, d);
+  if (J.ndict > 0) {
+    const DictMat mat{J.dsptr, J.wcol, J.dict, J.diag};
+    spmm_body<KP, kDotPhi, kUnroll>(J.n, mat, alpha, beta, sa, groups, J.phi, lb, J.nblk, d);
+  } else {
+    const PlainMat mat{J.sptr, J.col, J.val};
+    spmm_body<KP, kDotPhi, kUnroll>(J.n, mat, alpha, beta, sa, groups, J.phi, lb, J.nblk, d);
+  }
   block_reduce_store<KP, 1>(d, J.partial + (size_t)slot * J.nblk * KP, lb);
 }
 
@@ -333,7 +406,14 @@ const SpmmKnobs &knobs() {
 
 int xcd_q(int groups) { return knobs().xcd ? (groups + kXcds - 1) / kXcds : 0; }
 
-template <int KP, int U>
+bool use_dict(const SellDev &A) { return A.ndict > 0; }
+
+MatArgs mat_args(const SellDev &A) {
+  if (use_dict(A)) return MatArgs{A.dsptr, nullptr, nullptr, A.wcol, A.dict, A.diag, A.ndict};
+  return MatArgs{A.sptr, A.col, A.val, nullptr, nullptr, nullptr, 0};
+}
+
+template <int KP, int U, bool DICT>
 int launch_spmm_kpu(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, const SetArgs &sa,
                     int dot_mode, const double2 *phi, double2 *partial, int *grid_used) {
   constexpr int RPB = 256 / KP;
@@ -346,18 +426,19 @@ int launch_spmm_kpu(hipStream_t st, const SellDev &A, double2 alpha, double2 bet
     if (q > 0) grid = q * kXcds;
   }
   if (grid_used) *grid_used = grid;
+  const MatArgs m = mat_args(A);
   switch (dot_mode) {
     case kDotNone:
-      hipLaunchKernelGGL((k_spmm<KP, kDotNone, U>), dim3(grid), dim3(256), 0, st, A.n, A.sptr, A.col, A.val,
-                         alpha, beta, sa, groups, phi, partial, q);
+      hipLaunchKernelGGL((k_spmm<KP, kDotNone, U, DICT>), dim3(grid), dim3(256), 0, st, A.n, m, alpha, beta, sa,
+                         groups, phi, partial, q);
       break;
     case kDotPhi:
-      hipLaunchKernelGGL((k_spmm<KP, kDotPhi, U>), dim3(grid), dim3(256), 0, st, A.n, A.sptr, A.col, A.val,
-                         alpha, beta, sa, groups, phi, partial, 0);
+      hipLaunchKernelGGL((k_spmm<KP, kDotPhi, U, DICT>), dim3(grid), dim3(256), 0, st, A.n, m, alpha, beta, sa,
+                         groups, phi, partial, 0);
       break;
     case kDotDoubling:
-      hipLaunchKernelGGL((k_spmm<KP, kDotDoubling, U>), dim3(grid), dim3(256), 0, st, A.n, A.sptr, A.col, A.val,
-                         alpha, beta, sa, groups, phi, partial, 0);
+      hipLaunchKernelGGL((k_spmm<KP, kDotDoubling, U, DICT>), dim3(grid), dim3(256), 0, st, A.n, m, alpha, beta, sa,
+                         groups, phi, partial, 0);
       break;
     default:
       set_error("launch_spmm: bad dot mode %d", dot_mode);
@@ -369,13 +450,28 @@ int launch_spmm_kpu(hipStream_t st, const SellDev &A, double2 alpha, double2 bet
 
 // Unroll width: the smallest of 4/8/12/16 covering the widest slice (wider
 // operators loop over chunks of 16).
+int unroll_for(int wmax) { return wmax <= 4 ? 4 : wmax <= 8 ? 8 : wmax <= 12 ? 12 : 16; }
+
 template <int KP>
 int launch_spmm_kp(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, const SetArgs &sa,
                    int dot_mode, const double2 *phi, double2 *partial, int *grid_used) {
-  if (A.wmax <= 4) return launch_spmm_kpu<KP, 4>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
-  if (A.wmax <= 8) return launch_spmm_kpu<KP, 8>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
-  if (A.wmax <= 12) return launch_spmm_kpu<KP, 12>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
-  return launch_spmm_kpu<KP, 16>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
+#define LQ_KPU(UV, DV) \
+  return launch_spmm_kpu<KP, UV, DV>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used)
+  if (use_dict(A)) {
+    switch (unroll_for(A.dwmax)) {
+      case 4: LQ_KPU(4, true);
+      case 8: LQ_KPU(8, true);
+      case 12: LQ_KPU(12, true);
+      default: LQ_KPU(16, true);
+    }
+  }
+  switch (unroll_for(A.wmax)) {
+    case 4: LQ_KPU(4, false);
+    case 8: LQ_KPU(8, false);
+    case 12: LQ_KPU(12, false);
+    default: LQ_KPU(16, false);
+  }
+#undef LQ_KPU
 }
 
 }  // namespace

@@ -217,3 +217,34 @@ def test_bad_arguments_raise(ctx):
         lq.SparseMatrix(ctx, 3, np.array([0, 1, 2, 2]), np.array([0, 5]), np.ones(2))
     with pytest.raises(lq.LinqtError):
         lq.SparseMatrix(ctx, 3, np.array([0, 1, 2, 9]), np.array([0, 1]), np.ones(2))
+
+
+def test_dictionary_form_matches_plain_form(ctx, monkeypatch):
+    """Lattice operators get the dictionary form (value table + packed
+    columns, separate diagonal); it must agree with the plain form."""
+    ops = [lattices.square_anderson(20, W=1.0, seed=3), lattices.kane_mele(6, W=1.0, seed=4)]
+    H, Vx, Vy = lattices.haldane_honeycomb(7, seed=5)
+    ops += [H, Vx, Vy]
+    rng = np.random.default_rng(9)
+    for m in ops:
+        say("dict vs plain", m.n)
+        monkeypatch.setenv("LQ_SPMM_DICT", "0")
+        P = lq.SparseMatrix.from_csr(ctx, m)
+        monkeypatch.setenv("LQ_SPMM_DICT", "1")
+        D = lq.SparseMatrix.from_csr(ctx, m)
+        for k in (1, 3, 8):
+            x = rng.normal(size=(m.n, k)) + 1j * rng.normal(size=(m.n, k))
+            y = rng.normal(size=(m.n, k)) + 1j * rng.normal(size=(m.n, k))
+            a, b = 0.7 - 0.2j, -1.0
+            assert rel_inf(D.Multiply(a, x, b, y), P.Multiply(a, x, b, y)) < 1e-14
+        Dd = dense_herm(m.n, m.rowptr, m.col, m.val)
+        x = rng.normal(size=(m.n, 2)) + 1j * rng.normal(size=(m.n, 2))
+        assert rel_inf(D.Multiply(1.0, x), Dd @ x) < 1e-13
+    # rescaled operator and the 1D solver on both forms
+    m = lattices.square_anderson(24, W=1.0, seed=6)
+    monkeypatch.setenv("LQ_SPMM_DICT", "0")
+    P = lq.SparseMatrix.from_csr(ctx, m).Rescale(1 / 4.6, 0.1)
+    monkeypatch.setenv("LQ_SPMM_DICT", "1")
+    D = lq.SparseMatrix.from_csr(ctx, m).Rescale(1 / 4.6, 0.1)
+    phi = lattices.random_phase_states(m.n, 3, seed=1)
+    assert rel_inf(lq.spectral_moments(D, None, phi, 200), lq.spectral_moments(P, None, phi, 200)) < 1e-12

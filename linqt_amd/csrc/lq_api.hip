@@ -69,7 +69,7 @@ int time_begin(lq_ctx ctx, int cat, hipEvent_t *ev) {
   return LQ_OK;
 }
 
-int time_end(lq_ctx ctx, int cat, hipEvent_t ev, double amount) {
+int time_end(lq_ctx ctx, int cat, hipEvent_t ev, double amount, int launches) {
   if (!ctx->timing || !ev) return LQ_OK;
   if (ctx->event_pool.empty()) {
     hipEvent_t e;
@@ -79,8 +79,21 @@ int time_end(lq_ctx ctx, int cat, hipEvent_t ev, double amount) {
   hipEvent_t b = ctx->event_pool.back();
   ctx->event_pool.pop_back();
   LQ_HIP_CHECK(hipEventRecord(b, ctx->stream));
-  ctx->pending.push_back({cat, ev, b, amount});
+  ctx->pending.push_back({cat, ev, b, amount, launches});
   return LQ_OK;
+}
+
+int spmv_loop_begin(lq_ctx ctx) {
+  ctx->loop_bytes = 0.0;
+  ctx->loop_launches = 0;
+  LQ_TRY(time_begin(ctx, kCatSpmv, &ctx->loop_ev));
+  ctx->spmv_loop = 1;
+  return LQ_OK;
+}
+
+int spmv_loop_end(lq_ctx ctx) {
+  ctx->spmv_loop = 0;
+  return time_end(ctx, kCatSpmv, ctx->loop_ev, ctx->loop_bytes, ctx->loop_launches);
 }
 
 int collect_timing(lq_ctx ctx) {
@@ -93,12 +106,12 @@ int collect_timing(lq_ctx ctx) {
       case kCatSpmv:
         ctx->tsum.spmv_ms += ms;
         ctx->tsum.spmv_bytes += p.bytes_or_flops;
-        ctx->tsum.spmv_launches++;
+        ctx->tsum.spmv_launches += p.launches;
         break;
       case kCatContract:
         ctx->tsum.contract_ms += ms;
         ctx->tsum.contract_flops += p.bytes_or_flops;
-        ctx->tsum.contract_launches++;
+        ctx->tsum.contract_launches += p.launches;
         break;
       default:
         ctx->tsum.post_ms += ms;
@@ -321,15 +334,27 @@ long long spmv_bytes(const lq_csr A, int k, bool with_z) {
   return 20LL * (long long)A->col.size() + 4LL * (A->n + 1) + (with_z ? 48LL : 32LL) * k * (long long)A->n;
 }
 
-// One timed SpMM launch.
+// Closes an SpMM timing loop left open by an early error return.
+struct SpmvLoopGuard {
+  lq_ctx ctx;
+  ~SpmvLoopGuard() { ctx->spmv_loop = 0; }
+};
+
+// One SpMM launch, timed on its own or as part of an open SpMM loop.
 int spmm(lq_ctx ctx, lq_csr A, double2 alpha, double2 beta, const VecSet *sets, int nsets, int dot_mode = kDotNone,
          const double2 *phi = nullptr, double2 *partial = nullptr) {
+  int k = 0;
+  for (int s = 0; s < nsets; ++s) k += sets[s].k;
+  const double bytes = (double)spmv_bytes(A, k, sets[0].z != nullptr);
+  if (ctx->spmv_loop) {
+    ctx->loop_bytes += bytes;
+    ctx->loop_launches++;
+    return launch_spmm(ctx->stream, A->dev, alpha, beta, sets, nsets, dot_mode, phi, partial, nullptr);
+  }
   hipEvent_t ev;
   LQ_TRY(time_begin(ctx, kCatSpmv, &ev));
   LQ_TRY(launch_spmm(ctx->stream, A->dev, alpha, beta, sets, nsets, dot_mode, phi, partial, nullptr));
-  int k = 0;
-  for (int s = 0; s < nsets; ++s) k += sets[s].k;
-  return time_end(ctx, kCatSpmv, ev, (double)spmv_bytes(A, k, sets[0].z != nullptr));
+  return time_end(ctx, kCatSpmv, ev, bytes);
 }
 
 }  // namespace
@@ -438,6 +463,8 @@ int pipe2d_group(Pipe2D &P, const double *phi, int r0, int g, bool accumulate) {
   auto L = [&](int m) { return Ls + (size_t)m * vec; };
   auto RS = [&](int o, int m) { return Rs + ((size_t)o * P.MR + m) * vec; };
 
+  SpmvLoopGuard guard{ctx};
+  LQ_TRY(spmv_loop_begin(ctx));
   // T0 of the left recursion: OPL * phi
   if (P.OPL) {
     VecSet s{phi_d, nullptr, L(0), g};
@@ -481,6 +508,7 @@ int pipe2d_group(Pipe2D &P, const double *phi, int r0, int g, bool accumulate) {
       LQ_TRY(apply_opr(Rcur, m + 1));
     }
   }
+  LQ_TRY(spmv_loop_end(ctx));
   // raw_o (+)= L^H R^o
   for (int o = 0; o < P.nopr; ++o)
     LQ_TRY(contract_table(ctx, P.precision, Ls, (long long)vec, P.ML, RS(o, 0), (long long)vec, P.MR, (long long)vec,
@@ -489,10 +517,17 @@ int pipe2d_group(Pipe2D &P, const double *phi, int r0, int g, bool accumulate) {
 }
 
 // States per group: as many as fit (vectors per SpMM row <= 64 in total).
-int choose_group(int R, int n, int ML, int MR, int nopr, int precision) {
+int choose_group(lq_ctx ctx, int R, int n, int ML, int MR, int nopr, int precision) {
   size_t per_state = ((size_t)ML + (size_t)MR * nopr + 3) * (size_t)n * sizeof(double2);
   if (precision != LQ_PREC_FP64) per_state += ((size_t)ML + MR) * 2 * precision * (size_t)n;  // digit planes
+  // the pipeline's own buffers from earlier calls are reused (or freed before
+  // they grow), so they count as available: the group size, and with it the
+  // order of the sums, then does not depend on what ran before
   size_t budget = free_device_bytes();
+  for (const char *name : {"phi", "Lstore", "Rwork", "Rstore", "cwork", "ozQA", "ozEA", "ozQB", "ozEB"}) {
+    auto it = ctx->bufs.find(name);
+    if (it != ctx->bufs.end()) budget += it->second.bytes;
+  }
   budget = budget > ((size_t)4 << 30) ? budget - ((size_t)4 << 30) : budget / 2;  // keep head room
   int g = (int)std::min<size_t>((size_t)R, per_state ? budget / per_state : (size_t)R);
   g = std::max(1, std::min(g, 32));
@@ -501,7 +536,7 @@ int choose_group(int R, int n, int ML, int MR, int nopr, int precision) {
 }
 
 int run_pipe2d(Pipe2D &P, const double *phi, int R) {
-  const int g = choose_group(R, P.n, P.ML, P.MR, P.nopr, P.precision);
+  const int g = choose_group(P.ctx, R, P.n, P.ML, P.MR, P.nopr, P.precision);
   for (int r0 = 0; r0 < R; r0 += g) {
     const int gg = std::min(g, R - r0);
     LQ_TRY(pipe2d_group(P, phi, r0, gg, r0 > 0));
@@ -851,6 +886,8 @@ int lq_spectral_moments(lq_csr H, lq_csr OP, const double *phi, int R, int M, in
       LQ_TRY(ctx_buf(ctx, "sums0", (size_t)M * kp * sizeof(double2), (void **)&sums0));
       const size_t slot = (size_t)grid * kp;
       LQ_TRY(launch_dot(ctx->stream, n, gg, phi_d, A, part0, nullptr));
+      SpmvLoopGuard guard{ctx};
+      LQ_TRY(spmv_loop_begin(ctx));
       if (M > 1) {
         VecSet s{A, nullptr, B, gg};
         LQ_TRY(spmm(ctx, H, one, zero, &s, 1, kDotPhi, phi_d, part0 + slot));
@@ -861,6 +898,7 @@ int lq_spectral_moments(lq_csr H, lq_csr OP, const double *phi, int R, int M, in
         LQ_TRY(spmm(ctx, H, two, mone, &s, 1, kDotPhi, phi_d, part0 + (size_t)(m + 1) * slot));
         std::swap(A, B);
       }
+      LQ_TRY(spmv_loop_end(ctx));
       hipEvent_t ev;
       LQ_TRY(time_begin(ctx, kCatPost, &ev));
       LQ_TRY(launch_reduce_partials(ctx->stream, part0, M, grid, kp, sums0));
@@ -887,12 +925,15 @@ int lq_spectral_moments(lq_csr H, lq_csr OP, const double *phi, int R, int M, in
         LQ_TRY(spmm(ctx, H, one, zero, &s, 1));
         LQ_TRY(launch_dot(ctx->stream, n, gg, A, B, part0 + slot0, nullptr));
       }
+      SpmvLoopGuard guard{ctx};
+      LQ_TRY(spmv_loop_begin(ctx));
       for (int st = 1; st <= nsteps; ++st) {
         // x = T_st (B), z = y = T_st-1 (A): A <- T_st+1
         VecSet s{B, A, A, gg};
         LQ_TRY(spmm(ctx, H, two, mone, &s, 1, kDotDoubling, nullptr, part1 + (size_t)(st - 1) * slot1));
         std::swap(A, B);
       }
+      LQ_TRY(spmv_loop_end(ctx));
       hipEvent_t ev;
       LQ_TRY(time_begin(ctx, kCatPost, &ev));
       LQ_TRY(launch_reduce_partials(ctx->stream, part0, 2, grid, kp, sums0));

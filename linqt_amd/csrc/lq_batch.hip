@@ -202,7 +202,7 @@ int run_chunk(lq_batch b) {
         const int active = std::min(L, j->d.t0 + j->d.steps - b->t);
         bytes += (double)active * (20.0 * j->H->col.size() + 4.0 * (j->n + 1) + 48.0 * j->R * (double)j->n);
       }
-      LQ_TRY(time_end(ctx, kCatSpmv, ev, bytes));
+      LQ_TRY(time_end(ctx, kCatSpmv, ev, bytes, L));
       if (timing) {
         LQ_HIP_CHECK(hipEventRecord(b->ev1, ctx->stream));
         LQ_HIP_CHECK(hipEventSynchronize(b->ev1));

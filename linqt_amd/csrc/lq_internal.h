@@ -169,7 +169,14 @@ struct lq_ctx_s {
     int cat;
     hipEvent_t a, b;
     double bytes_or_flops;
+    int launches;
   };
+  // SpMM loop timing: one event pair around a whole recursion loop (per-launch
+  // events would add their own overhead to kernels of a few microseconds)
+  int spmv_loop = 0;
+  hipEvent_t loop_ev = nullptr;
+  double loop_bytes = 0.0;
+  int loop_launches = 0;
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
   struct Buf {
@@ -212,6 +219,9 @@ int upload_states(lq_ctx ctx, const double *phi, int n, int r0, int g, double2 *
 // Context helpers (lq_api.hip).
 int ctx_buf(lq_ctx ctx, const char *name, size_t bytes, void **p);
 int time_begin(lq_ctx ctx, int cat, hipEvent_t *ev);
-int time_end(lq_ctx ctx, int cat, hipEvent_t ev, double amount);
+int time_end(lq_ctx ctx, int cat, hipEvent_t ev, double amount, int launches = 1);
+// Bracket a run of SpMM launches with one event pair (bytes and launches summed).
+int spmv_loop_begin(lq_ctx ctx);
+int spmv_loop_end(lq_ctx ctx);
 int collect_timing(lq_ctx ctx);
 }  // namespace lq

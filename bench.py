@@ -389,6 +389,17 @@ def main():
     except (OSError, ValueError, KeyError):
         traffic_tab = {}
     spmm_names = sorted((k for k in traffic_tab if k.startswith("k_spmm<")), key=lambda k: -traffic_tab[k])
+    sequential = None
+    if args.workload == "batch":
+        # the same jobs, one after another through the single-job solver (untimed in value)
+        t0 = time.perf_counter()
+        seq = [lq.spectral_moments(H, None, ds, sp["M"]) for sp, (H, _, ds) in zip(specs, mats)]
+        ctx.synchronize()
+        t_seq = time.perf_counter() - t0
+        same = all(np.array_equal(a, b) for a, b in zip(seq, result))
+        sequential = {"ms": t_seq * 1e3, "speedup_of_batch": t_seq / (elapsed / args.steps),
+                      "bitwise_equal": bool(same)}
+        log(f"sequential single-job solves: {t_seq * 1e3:.1f} ms, batch identical: {same}")
     accuracy = None
     split = args.workload == "bastin" and args.precision != "fp64"
     if split:

@@ -1,6 +1,7 @@
 // lq_api.hip -- the C ABI (include/linqt_amd.h): contexts, operators and the
 // moment pipelines built from the kernels in lq_spmv / lq_contract / lq_post.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -127,6 +128,114 @@ int collect_timing(lq_ctx ctx) {
 namespace {
 
 // ---- operators ------------------------------------------------------------
+
+void free_fused(FusedDev &f) {
+  if (f.vidx) (void)hipFree(f.vidx);
+  for (int o = 0; o < 2; ++o) {
+    if (f.vdict[o]) (void)hipFree(f.vdict[o]);
+    if (f.vdiag[o]) (void)hipFree(f.vdiag[o]);
+  }
+  f = FusedDev{};
+}
+
+// Aligns up to two right operators to the stored slots of H's dictionary
+// form (see FusedDev).  *ok = false (nothing built) when H has no dictionary
+// form, an operator has an off-diagonal entry outside H's pattern, or more
+// than 256 distinct per-slot values.  Cached on H by operator id and version.
+int fused_prepare(lq_csr H, const lq_csr *V, int nv, bool *ok) {
+  *ok = false;
+  const char *env = getenv("LQ_FUSE_OPR");
+  if (env && env[0] == '0') return LQ_OK;
+  if (nv < 1 || nv > 2 || H->dev.ndict == 0) return LQ_OK;
+  for (int o = 0; o < nv; ++o)
+    if (!V[o] || V[o]->n != H->n || V[o]->ctx != H->ctx) return LQ_OK;
+  FusedDev &f = H->fused;
+  bool hit = f.nv == nv && f.hver == H->version;
+  for (int o = 0; o < nv && hit; ++o) hit = f.key_uid[o] == V[o]->uid && f.key_ver[o] == V[o]->version;
+  if (hit) {
+    *ok = true;
+    return LQ_OK;
+  }
+  const int n = H->n;
+  const int nslices = (n + kSlice - 1) / kSlice;
+  std::vector<int> offd(n, 0);
+  for (int i = 0; i < n; ++i)
+    for (int q = H->rowptr[i]; q < H->rowptr[i + 1]; ++q) offd[i] += H->col[q] != i;
+  std::vector<long long> sptr(nslices + 1, 0);
+  for (int sl = 0; sl < nslices; ++sl) {
+    int w = 0;
+    for (int r = 0; r < kSlice; ++r) {
+      const int i = sl * kSlice + r;
+      if (i < n) w = std::max(w, offd[i]);
+    }
+    sptr[sl + 1] = sptr[sl] + (long long)w * kSlice;
+  }
+  if (sptr[nslices] != H->dev.dstored) return LQ_OK;
+  const size_t stored = (size_t)sptr[nslices];
+  std::vector<unsigned short> vidx(stored > 0 ? stored : 1, 0);
+  std::vector<std::vector<double2>> dicts(nv), diags(nv);
+  std::vector<bool> has_diag(nv, false);
+  std::vector<std::pair<int, int>> slot_of;  // (column, slot position in row) for one row
+  for (int o = 0; o < nv; ++o) {
+    const lq_csr A = V[o];
+    std::vector<std::complex<double>> sv(stored > 0 ? stored : 1, 0.0);
+    std::vector<double2> dg(n, d2(0.0, 0.0));
+    for (int i = 0; i < n; ++i) {
+      slot_of.clear();
+      int j = 0;
+      for (int q = H->rowptr[i]; q < H->rowptr[i + 1]; ++q)
+        if (H->col[q] != i) slot_of.emplace_back(H->col[q], j++);
+      std::sort(slot_of.begin(), slot_of.end());
+      const int sl = i / kSlice, r = i % kSlice;
+      for (int q = A->rowptr[i]; q < A->rowptr[i + 1]; ++q) {
+        const int c = A->col[q];
+        if (c == i) {
+          dg[i].x += A->val[q].real();
+          dg[i].y += A->val[q].imag();
+          has_diag[o] = true;
+          continue;
+        }
+        auto it = std::lower_bound(slot_of.begin(), slot_of.end(), std::make_pair(c, -1));
+        if (it == slot_of.end() || it->first != c) return LQ_OK;  // outside H's pattern
+        sv[(size_t)sptr[sl] + (size_t)it->second * kSlice + r] += A->val[q];
+      }
+    }
+    std::map<std::pair<double, double>, int> index;
+    for (size_t p = 0; p < stored; ++p) {
+      const auto key = std::make_pair(sv[p].real(), sv[p].imag());
+      auto it = index.find(key);
+      int id;
+      if (it == index.end()) {
+        if (index.size() == 256) return LQ_OK;
+        id = (int)index.size();
+        index.emplace(key, id);
+        dicts[o].push_back(d2(key.first, key.second));
+      } else {
+        id = it->second;
+      }
+      vidx[p] |= (unsigned short)(id << (8 * o));
+    }
+    if (dicts[o].empty()) dicts[o].push_back(d2(0.0, 0.0));
+    diags[o] = std::move(dg);
+  }
+  free_fused(f);
+  LQ_HIP_CHECK(hipMalloc(&f.vidx, sizeof(unsigned short) * vidx.size()));
+  LQ_HIP_CHECK(hipMemcpy(f.vidx, vidx.data(), sizeof(unsigned short) * vidx.size(), hipMemcpyHostToDevice));
+  for (int o = 0; o < nv; ++o) {
+    LQ_HIP_CHECK(hipMalloc(&f.vdict[o], sizeof(double2) * dicts[o].size()));
+    LQ_HIP_CHECK(hipMemcpy(f.vdict[o], dicts[o].data(), sizeof(double2) * dicts[o].size(), hipMemcpyHostToDevice));
+    if (has_diag[o]) {
+      LQ_HIP_CHECK(hipMalloc(&f.vdiag[o], sizeof(double2) * n));
+      LQ_HIP_CHECK(hipMemcpy(f.vdiag[o], diags[o].data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    }
+    f.key_uid[o] = V[o]->uid;
+    f.key_ver[o] = V[o]->version;
+  }
+  f.nv = nv;
+  f.hver = H->version;
+  *ok = true;
+  return LQ_OK;
+}
 
 void free_dev(SellDev &d) {
   if (d.sptr) (void)hipFree(d.sptr);
@@ -341,19 +450,22 @@ struct SpmvLoopGuard {
 };
 
 // One SpMM launch, timed on its own or as part of an open SpMM loop.
+// extra_bytes: algorithmic bytes of fused right-operator applies (counted as
+// the separate applies they replace).
 int spmm(lq_ctx ctx, lq_csr A, double2 alpha, double2 beta, const VecSet *sets, int nsets, int dot_mode = kDotNone,
-         const double2 *phi = nullptr, double2 *partial = nullptr) {
+         const double2 *phi = nullptr, double2 *partial = nullptr, const FusedArgs *fused = nullptr,
+         double extra_bytes = 0.0) {
   int k = 0;
   for (int s = 0; s < nsets; ++s) k += sets[s].k;
-  const double bytes = (double)spmv_bytes(A, k, sets[0].z != nullptr);
+  const double bytes = (double)spmv_bytes(A, k, sets[0].z != nullptr) + extra_bytes;
   if (ctx->spmv_loop) {
     ctx->loop_bytes += bytes;
     ctx->loop_launches++;
-    return launch_spmm(ctx->stream, A->dev, alpha, beta, sets, nsets, dot_mode, phi, partial, nullptr);
+    return launch_spmm(ctx->stream, A->dev, alpha, beta, sets, nsets, dot_mode, phi, partial, nullptr, fused);
   }
   hipEvent_t ev;
   LQ_TRY(time_begin(ctx, kCatSpmv, &ev));
-  LQ_TRY(launch_spmm(ctx->stream, A->dev, alpha, beta, sets, nsets, dot_mode, phi, partial, nullptr));
+  LQ_TRY(launch_spmm(ctx->stream, A->dev, alpha, beta, sets, nsets, dot_mode, phi, partial, nullptr, fused));
   return time_end(ctx, kCatSpmv, ev, bytes);
 }
 
@@ -484,9 +596,50 @@ int pipe2d_group(Pipe2D &P, const double *phi, int r0, int g, bool accumulate) {
     }
     return LQ_OK;
   };
+  const int Mmax = std::max(P.ML, P.MR);
+  // Right-operator applies fused into the H step when every OPR's pattern lies
+  // inside H's: the step that reads T_m as x also writes OPR_o T_m.
+  bool fuse = false;
+  if (P.nopr <= 2 && P.MR > 1) LQ_TRY(fused_prepare(P.H, P.OPR, P.nopr, &fuse));
+  if (fuse) {
+    FusedArgs fa;
+    fa.nv = P.nopr;
+    fa.vidx = P.H->fused.vidx;
+    double vbytes = 0.0;
+    for (int o = 0; o < P.nopr; ++o) {
+      fa.vdict[o] = P.H->fused.vdict[o];
+      fa.vdiag[o] = P.H->fused.vdiag[o];
+      vbytes += (double)spmv_bytes(P.OPR[o], g, false);
+    }
+    for (int m = 0; m + 1 < Mmax; ++m) {
+      VecSet sets[2];
+      int ns = 0;
+      if (m + 1 < P.ML) sets[ns++] = m == 0 ? VecSet{L(0), nullptr, L(1), g} : VecSet{L(m), L(m - 1), L(m + 1), g};
+      const bool rstep = m + 1 < P.MR;
+      if (rstep) {
+        // x = T_m (Rcur, or T_0 in Rprev at m = 0); V_o T_m -> RS(o, m)
+        sets[ns++] = m == 0 ? VecSet{Rprev, nullptr, Rcur, g} : VecSet{Rcur, Rprev, Rprev, g};
+        fa.set = ns - 1;
+        for (int o = 0; o < P.nopr; ++o) fa.yv[o] = RS(o, m);
+      }
+      if (m == 0)
+        LQ_TRY(spmm(ctx, P.H, one, zero, sets, ns, kDotNone, nullptr, nullptr, rstep ? &fa : nullptr,
+                    rstep ? vbytes : 0.0));
+      else
+        LQ_TRY(spmm(ctx, P.H, two, mone, sets, ns, kDotNone, nullptr, nullptr, rstep ? &fa : nullptr,
+                    rstep ? vbytes : 0.0));
+      if (rstep && m > 0) std::swap(Rcur, Rprev);
+    }
+    // the last right slot, T_{MR-1}, is not the x of any step
+    LQ_TRY(apply_opr(P.MR > 1 ? Rcur : Rprev, P.MR - 1));
+    LQ_TRY(spmv_loop_end(ctx));
+    for (int o = 0; o < P.nopr; ++o)
+      LQ_TRY(contract_table(ctx, P.precision, Ls, (long long)vec, P.ML, RS(o, 0), (long long)vec, P.MR,
+                            (long long)vec, P.raw + (size_t)o * P.ML * P.MR, P.MR, accumulate, o == 0));
+    return LQ_OK;
+  }
   if (P.MR > 0) LQ_TRY(apply_opr(Rprev, 0));
   // T1 = H~ T0 on both sides
-  const int Mmax = std::max(P.ML, P.MR);
   if (Mmax > 1) {
     VecSet sets[2];
     int ns = 0;
@@ -746,8 +899,10 @@ int lq_csr_create(lq_ctx ctx, int n, int nnz, const int *rowptr, const int *col,
     }
   }
   DeviceGuard g(ctx->device);
+  static std::atomic<unsigned long long> next_uid{1};
   lq_csr A = new lq_csr_s();
   A->ctx = ctx;
+  A->uid = next_uid++;
   A->n = n;
   A->fill = fill;
   set_entries(A, e);
@@ -766,6 +921,7 @@ int lq_csr_destroy(lq_csr A) {
   DeviceGuard g(A->ctx->device);
   (void)hipStreamSynchronize(A->ctx->stream);
   free_dev(A->dev);
+  free_fused(A->fused);
   delete A;
   return LQ_OK;
 }
@@ -796,6 +952,7 @@ int lq_csr_rescale(lq_csr A, double a_re, double a_im, double b_re, double b_im)
   DeviceGuard g(A->ctx->device);
   LQ_HIP_CHECK(hipStreamSynchronize(A->ctx->stream));
   set_entries(A, e);
+  A->version++;
   return upload_sell(A);
 }
 

@@ -81,6 +81,30 @@ enum DotMode {
   kDotDoubling = 2  // partials = sum conj(x_i)x_i and sum conj(x_i) y_i (row's own x)
 };
 
+// Right operators applied inside the recursion step of H (at most 2; their
+// patterns lie inside H's): per stored slot of H's dictionary form, byte o of
+// vidx indexes operator o's value table; vdiag[o] is its diagonal (or null).
+struct FusedDev {
+  int nv = 0;
+  unsigned short *vidx = nullptr;
+  double2 *vdict[2] = {nullptr, nullptr};
+  double2 *vdiag[2] = {nullptr, nullptr};
+  unsigned long long key_uid[2] = {0, 0};
+  int key_ver[2] = {-1, -1};
+  int hver = -1;
+};
+
+// A launch's fused right-operator products: yv[o] = V_o x for the lanes of
+// vector set `set` (same [n][k] layout as that set's y).
+struct FusedArgs {
+  int nv = 0;
+  int set = 1;
+  const unsigned short *vidx = nullptr;
+  const double2 *vdict[2] = {nullptr, nullptr};
+  const double2 *vdiag[2] = {nullptr, nullptr};
+  double2 *yv[2] = {nullptr, nullptr};
+};
+
 // One job of a batched recursion launch (device table entry).
 struct BatchJobDev {
   const int *sptr;
@@ -101,7 +125,8 @@ struct BatchJobDev {
 
 // Launchers (lq_spmv.hip).  Return LQ_OK or an error code.
 int launch_spmm(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, const VecSet *sets,
-                int nsets, int dot_mode, const double2 *phi, double2 *partial, int *grid_used);
+                int nsets, int dot_mode, const double2 *phi, double2 *partial, int *grid_used,
+                const FusedArgs *fused = nullptr);
 int spmm_dot_grid(int n, int kpad);  // number of blocks a dot-mode launch uses
 int launch_dot(hipStream_t st, int n, int k, const double2 *a, const double2 *b, double2 *partial,
                int *grid_used);
@@ -198,6 +223,9 @@ struct lq_csr_s {
   std::vector<std::complex<double>> val;
   std::vector<signed char> kind;  // 0: as stored (upper or general), 1: mirrored conj, 2: diagonal
   lq::SellDev dev;
+  unsigned long long uid = 0;  // unique per operator of the process
+  int version = 0;             // bumped by every rescale
+  lq::FusedDev fused;          // right operators fused into this operator's recursion step
 };
 
 namespace lq {

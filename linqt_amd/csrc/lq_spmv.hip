@@ -98,10 +98,12 @@ struct DictMat {
 
 // Rows of row groups g0, g0 + gstep, ... (RPB = 256 / KP rows each) of one
 // launch; dot contributions are accumulated into d.
-template <int KP, int DOT, int kUnroll, class Mat>
+// NV > 0: the lanes of vector set fa.set also form yv[o] = V_o x (o < NV)
+// from the same gathers, V_o's values indexed per stored slot by fa.vidx.
+template <int KP, int DOT, int kUnroll, class Mat, int NV = 0>
 __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, double2 beta, const SetArgs &sa,
                                           int nrowgroups, const double2 *__restrict__ phi, int g0, int gstep,
-                                          double2 (&d)[2]) {
+                                          double2 (&d)[2], const FusedArgs &fa = FusedArgs{}) {
   constexpr int RPB = 256 / KP;
   const int tid = threadIdx.x;
   const int v = tid % KP;
@@ -112,6 +114,7 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
   const double2 *__restrict__ x = sa.x[set];
   const double2 *z = sa.z[set];
   double2 *y = sa.y[set];
+  const bool vlane = NV > 0 && set == fa.set;
   for (int g = g0; g < nrowgroups; g += gstep) {
     const int row = g * RPB + tid / KP;
     if (row < n && lane_on) {
@@ -123,16 +126,28 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
       // the separate diagonal (dictionary form) is loaded with the first
       // entries so it adds no memory round trip of its own
       double2 dv = make_double2(0.0, 0.0), xo = make_double2(0.0, 0.0);
-      if (mat.has_diag()) {
-        dv = mat.diag(row);
-        xo = x[o];
+      double2 vdv[NV > 0 ? NV : 1];
+      bool vdg = false;
+#pragma unroll
+      for (int q = 0; q < NV; ++q) {
+        vdv[q] = make_double2(0.0, 0.0);
+        if (vlane && fa.vdiag[q]) {
+          vdv[q] = fa.vdiag[q][row];
+          vdg = true;
+        }
       }
+      if (mat.has_diag()) dv = mat.diag(row);
+      if (mat.has_diag() || vdg) xo = x[o];
       double ar = 0.0, ai = 0.0;
+      double vr[NV > 0 ? NV : 1], vi[NV > 0 ? NV : 1];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) vr[q] = vi[q] = 0.0;
       // Chunks of kUnroll entries: all column/value loads of a chunk are in
       // flight together, then all of its gathers, so a lattice row costs two
       // memory latencies instead of one per entry.
       for (int j0 = 0; j0 < w; j0 += kUnroll) {
         unsigned wd[kUnroll];
+        unsigned vw[NV > 0 ? kUnroll : 1];
         int c[kUnroll];
         double2 a[kUnroll];
 #pragma unroll
@@ -140,6 +155,7 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
           wd[u] = 0u;
           a[u] = make_double2(0.0, 0.0);
           if (j0 + u < w) mat.load(p0 + (size_t)(j0 + u) * kSlice, wd[u], a[u]);
+          if (NV > 0) vw[u] = (vlane && j0 + u < w) ? (unsigned)fa.vidx[p0 + (size_t)(j0 + u) * kSlice] : 0u;
         }
         double2 xv[kUnroll];
 #pragma unroll
@@ -157,6 +173,31 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
           ar = fma(-a[u].y, xv[u].y, ar);
           ai = fma(a[u].x, xv[u].y, ai);
           ai = fma(a[u].y, xv[u].x, ai);
+        }
+        if (vlane) {
+#pragma unroll
+          for (int u = 0; u < kUnroll; ++u) {
+            if (j0 + u < w) {
+#pragma unroll
+              for (int q = 0; q < NV; ++q) {
+                const double2 b = fa.vdict[q][(vw[u] >> (8 * q)) & 0xffu];
+                vr[q] = fma(b.x, xv[u].x, vr[q]);
+                vr[q] = fma(-b.y, xv[u].y, vr[q]);
+                vi[q] = fma(b.x, xv[u].y, vi[q]);
+                vi[q] = fma(b.y, xv[u].x, vi[q]);
+              }
+            }
+          }
+        }
+      }
+      if (vlane) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+          vr[q] = fma(vdv[q].x, xo.x, vr[q]);
+          vr[q] = fma(-vdv[q].y, xo.y, vr[q]);
+          vi[q] = fma(vdv[q].x, xo.y, vi[q]);
+          vi[q] = fma(vdv[q].y, xo.x, vi[q]);
+          fa.yv[q][o] = make_double2(vr[q], vi[q]);
         }
       }
       if (mat.has_diag()) {
@@ -200,10 +241,10 @@ struct MatArgs {
   int ndict;  // > 0: dictionary form
 };
 
-template <int KP, int DOT, int kUnroll, bool DICT>
+template <int KP, int DOT, int kUnroll, bool DICT, int NV = 0>
 __global__ __launch_bounds__(256) void k_spmm(int n, MatArgs m, double2 alpha, double2 beta, SetArgs sa,
                                               int nrowgroups, const double2 *__restrict__ phi,
-                                              double2 *__restrict__ partial, int q) {
+                                              double2 *__restrict__ partial, int q, FusedArgs fa) {
   constexpr int NDOT = (DOT == kDotDoubling) ? 2 : 1;
   double2 d[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
   // q > 0: one row group per block, XCD-remapped; q == 0: grid-stride loop
@@ -211,7 +252,7 @@ __global__ __launch_bounds__(256) void k_spmm(int n, MatArgs m, double2 alpha, d
   const int gstep = q > 0 ? nrowgroups : gridDim.x;
   if (DICT) {
     const DictMat mat{m.sptr, m.wcol, m.dict, m.diag};
-    spmm_body<KP, DOT, kUnroll>(n, mat, alpha, beta, sa, nrowgroups, phi, g0, gstep, d);
+    spmm_body<KP, DOT, kUnroll, DictMat, NV>(n, mat, alpha, beta, sa, nrowgroups, phi, g0, gstep, d, fa);
   } else {
     const PlainMat mat{m.sptr, m.col, m.val};
     spmm_body<KP, DOT, kUnroll>(n, mat, alpha, beta, sa, nrowgroups, phi, g0, gstep, d);
@@ -415,7 +456,7 @@ MatArgs mat_args(const SellDev &A) {
 
 template <int KP, int U, bool DICT>
 int launch_spmm_kpu(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, const SetArgs &sa,
-                    int dot_mode, const double2 *phi, double2 *partial, int *grid_used) {
+                    int dot_mode, const double2 *phi, double2 *partial, int *grid_used, const FusedArgs &fa) {
   constexpr int RPB = 256 / KP;
   const int groups = (A.n + RPB - 1) / RPB;
   int grid = groups, q = 0;
@@ -427,18 +468,37 @@ int launch_spmm_kpu(hipStream_t st, const SellDev &A, double2 alpha, double2 bet
   }
   if (grid_used) *grid_used = grid;
   const MatArgs m = mat_args(A);
+  if (fa.nv > 0) {
+    if constexpr (DICT) {
+      if (dot_mode != kDotNone) {
+        set_error("launch_spmm: fused operators take no dot fusion");
+        return LQ_ERR_ARG;
+      }
+      if (fa.nv == 1)
+        hipLaunchKernelGGL((k_spmm<KP, kDotNone, U, true, 1>), dim3(grid), dim3(256), 0, st, A.n, m, alpha, beta,
+                           sa, groups, phi, partial, q, fa);
+      else
+        hipLaunchKernelGGL((k_spmm<KP, kDotNone, U, true, 2>), dim3(grid), dim3(256), 0, st, A.n, m, alpha, beta,
+                           sa, groups, phi, partial, q, fa);
+      LQ_HIP_CHECK(hipGetLastError());
+      return LQ_OK;
+    } else {
+      set_error("launch_spmm: fused operators need the dictionary form");
+      return LQ_ERR_ARG;
+    }
+  }
   switch (dot_mode) {
     case kDotNone:
       hipLaunchKernelGGL((k_spmm<KP, kDotNone, U, DICT>), dim3(grid), dim3(256), 0, st, A.n, m, alpha, beta, sa,
-                         groups, phi, partial, q);
+                         groups, phi, partial, q, fa);
       break;
     case kDotPhi:
       hipLaunchKernelGGL((k_spmm<KP, kDotPhi, U, DICT>), dim3(grid), dim3(256), 0, st, A.n, m, alpha, beta, sa,
-                         groups, phi, partial, 0);
+                         groups, phi, partial, 0, fa);
       break;
     case kDotDoubling:
       hipLaunchKernelGGL((k_spmm<KP, kDotDoubling, U, DICT>), dim3(grid), dim3(256), 0, st, A.n, m, alpha, beta, sa,
-                         groups, phi, partial, 0);
+                         groups, phi, partial, 0, fa);
       break;
     default:
       set_error("launch_spmm: bad dot mode %d", dot_mode);
@@ -454,9 +514,9 @@ int unroll_for(int wmax) { return wmax <= 4 ? 4 : wmax <= 8 ? 8 : wmax <= 12 ? 1
 
 template <int KP>
 int launch_spmm_kp(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, const SetArgs &sa,
-                   int dot_mode, const double2 *phi, double2 *partial, int *grid_used) {
+                   int dot_mode, const double2 *phi, double2 *partial, int *grid_used, const FusedArgs &fa) {
 #define LQ_KPU(UV, DV) \
-  return launch_spmm_kpu<KP, UV, DV>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used)
+  return launch_spmm_kpu<KP, UV, DV>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used, fa)
   if (use_dict(A)) {
     switch (unroll_for(A.dwmax)) {
       case 4: LQ_KPU(4, true);
@@ -483,7 +543,12 @@ int spmm_dot_grid(int n, int kpad) {
 }
 
 int launch_spmm(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, const VecSet *sets, int nsets,
-                int dot_mode, const double2 *phi, double2 *partial, int *grid_used) {
+                int dot_mode, const double2 *phi, double2 *partial, int *grid_used, const FusedArgs *fused) {
+  const FusedArgs fa = fused ? *fused : FusedArgs{};
+  if (fa.nv < 0 || fa.nv > 2 || (fa.nv > 0 && (fa.set < 0 || fa.set >= nsets || !fa.vidx))) {
+    set_error("launch_spmm: bad fused operator arguments");
+    return LQ_ERR_ARG;
+  }
   if (nsets < 1 || nsets > 2) {
     set_error("launch_spmm: nsets must be 1 or 2");
     return LQ_ERR_ARG;
@@ -513,7 +578,7 @@ int launch_spmm(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, c
   }
   // lane-per-row schedule: equal power-of-two widths per set, no dot fusion
   const int ks = sets[0].k;
-  const bool rows_ok = knobs().rows && dot_mode == kDotNone && (nsets == 1 || sets[1].k == ks) &&
+  const bool rows_ok = knobs().rows && fa.nv == 0 && dot_mode == kDotNone && (nsets == 1 || sets[1].k == ks) &&
                        (ks == 1 || ks == 2 || ks == 4 || ks == 8 || (ks == 16 && nsets == 1));
   if (rows_ok) {
     const int groups = (A.n + 255) / 256;
@@ -545,13 +610,13 @@ int launch_spmm(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, c
   }
   const int kp = pad_pow2(ktot);
   switch (kp) {
-    case 1: return launch_spmm_kp<1>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
-    case 2: return launch_spmm_kp<2>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
-    case 4: return launch_spmm_kp<4>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
-    case 8: return launch_spmm_kp<8>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
-    case 16: return launch_spmm_kp<16>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
-    case 32: return launch_spmm_kp<32>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
-    case 64: return launch_spmm_kp<64>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used);
+    case 1: return launch_spmm_kp<1>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used, fa);
+    case 2: return launch_spmm_kp<2>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used, fa);
+    case 4: return launch_spmm_kp<4>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used, fa);
+    case 8: return launch_spmm_kp<8>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used, fa);
+    case 16: return launch_spmm_kp<16>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used, fa);
+    case 32: return launch_spmm_kp<32>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used, fa);
+    case 64: return launch_spmm_kp<64>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used, fa);
     default:
       set_error("launch_spmm: %d vectors per row exceed 64", ktot);
       return LQ_ERR_ARG;

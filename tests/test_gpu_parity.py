@@ -248,3 +248,31 @@ def test_dictionary_form_matches_plain_form(ctx, monkeypatch):
     D = lq.SparseMatrix.from_csr(ctx, m).Rescale(1 / 4.6, 0.1)
     phi = lattices.random_phase_states(m.n, 3, seed=1)
     assert rel_inf(lq.spectral_moments(D, None, phi, 200), lq.spectral_moments(P, None, phi, 200)) < 1e-12
+
+
+def test_fused_right_operators_match_separate_applies(ctx, oracle, monkeypatch):
+    """V_x, V_y (patterns inside H's) are applied inside the H step; results
+    equal the separate applies, the oracle, and an operator outside H's
+    pattern falls back to separate applies."""
+    n, G, O = _honeycomb(ctx, oracle, 6)
+    phi = lattices.random_phase_states(n, 3, seed=11)
+    for formula in (0, 1):
+        say("fused kubo", formula)
+        monkeypatch.setenv("LQ_FUSE_OPR", "1")
+        fused = lq.kubo_fft(G[0], G[1], [G[1], G[2]], phi, 40, 80, formula)
+        monkeypatch.setenv("LQ_FUSE_OPR", "0")
+        sep = lq.kubo_fft(G[0], G[1], [G[1], G[2]], phi, 40, 80, formula)
+        assert rel_inf(fused, sep) < 1e-12
+        ref, _ = oracle.kubo_fft(O[0], O[1], O[2], phi, 40, 80, formula)
+        assert rel_inf(fused[1], ref) < 1e-10
+    monkeypatch.setenv("LQ_FUSE_OPR", "1")
+    one = lq.correlation_expansion_moments(G[0], G[1], G[2], phi, 24, 24)
+    ref = oracle.correlation_moments(O[0], O[1], O[2], phi, 24, 24)
+    assert rel_inf(one, ref) < 1e-10
+    # a right operator with entries outside H's pattern
+    rowptr, col, val = random_upper_csr(n, 0.02, seed=5)
+    X = lq.SparseMatrix(ctx, n, rowptr, col, val)
+    OX = oracle.Op(n, rowptr, col, val)
+    got = lq.correlation_expansion_moments(G[0], G[1], [G[2], X], phi, 16, 16)
+    assert rel_inf(got[1], oracle.correlation_moments(O[0], O[1], OX, phi, 16, 16)) < 1e-10
+    assert rel_inf(got[0], oracle.correlation_moments(O[0], O[1], O[2], phi, 16, 16)) < 1e-10

@@ -222,6 +222,7 @@ int fused_prepare(lq_csr H, const lq_csr *V, int nv, bool *ok) {
   LQ_HIP_CHECK(hipMalloc(&f.vidx, sizeof(unsigned short) * vidx.size()));
   LQ_HIP_CHECK(hipMemcpy(f.vidx, vidx.data(), sizeof(unsigned short) * vidx.size(), hipMemcpyHostToDevice));
   for (int o = 0; o < nv; ++o) {
+    f.nvdict[o] = (int)dicts[o].size();
     LQ_HIP_CHECK(hipMalloc(&f.vdict[o], sizeof(double2) * dicts[o].size()));
     LQ_HIP_CHECK(hipMemcpy(f.vdict[o], dicts[o].data(), sizeof(double2) * dicts[o].size(), hipMemcpyHostToDevice));
     if (has_diag[o]) {
@@ -608,6 +609,7 @@ int pipe2d_group(Pipe2D &P, const double *phi, int r0, int g, bool accumulate) {
     double vbytes = 0.0;
     for (int o = 0; o < P.nopr; ++o) {
       fa.vdict[o] = P.H->fused.vdict[o];
+      fa.nvdict[o] = P.H->fused.nvdict[o];
       fa.vdiag[o] = P.H->fused.vdiag[o];
       vbytes += (double)spmv_bytes(P.OPR[o], g, false);
     }

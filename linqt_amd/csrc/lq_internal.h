@@ -88,6 +88,7 @@ struct FusedDev {
   int nv = 0;
   unsigned short *vidx = nullptr;
   double2 *vdict[2] = {nullptr, nullptr};
+  int nvdict[2] = {0, 0};
   double2 *vdiag[2] = {nullptr, nullptr};
   unsigned long long key_uid[2] = {0, 0};
   int key_ver[2] = {-1, -1};
@@ -101,6 +102,7 @@ struct FusedArgs {
   int set = 1;
   const unsigned short *vidx = nullptr;
   const double2 *vdict[2] = {nullptr, nullptr};
+  int nvdict[2] = {0, 0};
   const double2 *vdiag[2] = {nullptr, nullptr};
   double2 *yv[2] = {nullptr, nullptr};
 };

@@ -85,7 +85,7 @@ struct PlainMat {
 struct DictMat {
   const int *__restrict__ sptr;
   const unsigned *__restrict__ wcol;
-  const double2 *__restrict__ dict;  // <= 256 entries: stays in the vector L1 cache
+  const double2 *dict;  // value table (LDS copy, or global in the batch kernel)
   const double2 *__restrict__ dg;
   __device__ __forceinline__ void load(size_t p, unsigned &w, double2 &) const { w = wcol[p]; }
   __device__ __forceinline__ void decode(unsigned w, int &c, double2 &a) const {
@@ -95,6 +95,16 @@ struct DictMat {
   __device__ __forceinline__ bool has_diag() const { return dg != nullptr; }
   __device__ __forceinline__ double2 diag(int row) const { return dg[row]; }
 };
+
+// Value tables live in LDS for the dictionary-form kernels: a lookup per lane
+// per entry from LDS instead of a 16-byte-per-lane load through the vector
+// memory path that also carries the gathers.  The grids of these kernels are
+// capped (grid-stride) so the one-off staging per block is amortised.
+constexpr int kDictGridCap = 2048;
+
+__device__ __forceinline__ void stage(double2 *dst, const double2 *__restrict__ src, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
 
 // Rows of row groups g0, g0 + gstep, ... (RPB = 256 / KP rows each) of one
 // launch; dot contributions are accumulated into d.
@@ -251,8 +261,18 @@ __global__ __launch_bounds__(256) void k_spmm(int n, MatArgs m, double2 alpha, d
   const int g0 = q > 0 ? xcd_group(blockIdx.x, q) : blockIdx.x;
   const int gstep = q > 0 ? nrowgroups : gridDim.x;
   if (DICT) {
-    const DictMat mat{m.sptr, m.wcol, m.dict, m.diag};
-    spmm_body<KP, DOT, kUnroll, DictMat, NV>(n, mat, alpha, beta, sa, nrowgroups, phi, g0, gstep, d, fa);
+    __shared__ double2 sdict[256];
+    __shared__ double2 svdict[NV > 0 ? NV : 1][NV > 0 ? 256 : 1];
+    stage(sdict, m.dict, m.ndict);
+    FusedArgs fl = fa;
+#pragma unroll
+    for (int o = 0; o < NV; ++o) {
+      stage(svdict[o], fa.vdict[o], fa.nvdict[o]);
+      fl.vdict[o] = svdict[o];
+    }
+    __syncthreads();
+    const DictMat mat{m.sptr, m.wcol, sdict, m.diag};
+    spmm_body<KP, DOT, kUnroll, DictMat, NV>(n, mat, alpha, beta, sa, nrowgroups, phi, g0, gstep, d, fl);
   } else {
     const PlainMat mat{m.sptr, m.col, m.val};
     spmm_body<KP, DOT, kUnroll>(n, mat, alpha, beta, sa, nrowgroups, phi, g0, gstep, d);
@@ -462,6 +482,8 @@ int launch_spmm_kpu(hipStream_t st, const SellDev &A, double2 alpha, double2 bet
   int grid = groups, q = 0;
   if (dot_mode != kDotNone) {
     grid = groups < kDotGridCap ? groups : kDotGridCap;
+  } else if (DICT) {
+    grid = groups < kDictGridCap ? groups : kDictGridCap;
   } else {
     q = xcd_q(groups);
     if (q > 0) grid = q * kXcds;

@@ -180,6 +180,26 @@ int lq_greenwood_fftgrid(lq_ctx ctx, const double *mu, int M0, int M1, int n, do
 int lq_bastin_fftgrid(lq_ctx ctx, const double *mu, int M0, int M1, int n, double W, double Ec,
                       int num_div, double *E, double *sigma);
 
+/* Linear-grid post-processors: the tools' grid is num_div energies
+ * x_i = -xb + i*2xb/(num_div-1) (xb = 0.99 for Bastin, 1 otherwise; the tools
+ * use num_div = 30 * max moments), E = a*x + Ec.  Rows written: num_div for
+ * LQ_LIN_SPECTRAL (mu: M0 moments, M1 ignored), num_div-1 otherwise (mu:
+ * M0 x M1).  Jackson-damped moments in, as the tools apply it first.  With
+ * xb = 1 the end points divide by sqrt(1 - x^2) = 0 and print NaN, and the
+ * running sums of BastinII and BastinSea are then NaN throughout, as in the
+ * reference. */
+enum lq_lingrid {
+  LQ_LIN_SPECTRAL = 0,    /* spectralFunctionFromChebmom.cpp:35-67 */
+  LQ_LIN_GREENWOOD = 1,   /* kuboGreenwoodFromChebmom.cpp:35-76 */
+  LQ_LIN_BASTIN = 2,      /* kuboBastinFromChebmom.cpp:35-86 */
+  LQ_LIN_BASTIN_I = 3,    /* kuboBastinIFromChebmom.cpp:35-88 */
+  LQ_LIN_BASTIN_II = 4,   /* kuboBastinIIFromChebmom.cpp:35-93 */
+  LQ_LIN_BASTIN_SEA = 5,  /* kuboBastinSeaFromChebmom.cpp:35-93 */
+  LQ_LIN_BASTIN_SURF = 6  /* kuboBastinSurfFromChebmom.cpp:35-86 */
+};
+int lq_linear_grid(lq_ctx ctx, int kind, const double *mu, int M0, int M1, int n, double W, double Ec,
+                   int num_div, double *E, double *out);
+
 /* ---- batched spectral-moment jobs ------------------------------------------ */
 /* Many SpectralMoments jobs (OP = identity, src/chebyshev_solver.cpp:176-200),
  * each with its own rescaled H, states and M, advanced together: one grouped

@@ -32,7 +32,7 @@ from ._lib import (LQ_ALGO_DIRECT, LQ_ALGO_DOUBLING, LQ_FILL_GENERAL, LQ_FILL_HE
                    LQ_PREC_INT8X6, LQ_PREC_INT8X7, LinqtError)
 
 __all__ = [
-    "Context", "SparseMatrix", "DeviceStates", "Batch", "spectral_moments", "correlation_expansion_moments", "kubo_fft", "kubo_post",
+    "Context", "SparseMatrix", "DeviceStates", "Batch", "linear_grid", "spectral_moments", "correlation_expansion_moments", "kubo_fft", "kubo_post",
     "jackson_1d", "jackson_2d", "dos_fftgrid", "greenwood_fftgrid", "bastin_fftgrid", "LinqtError",
     "LQ_FILL_HERMITIAN_UPPER", "LQ_FILL_GENERAL", "LQ_KUBO_GREENWOOD", "LQ_KUBO_BASTIN", "LQ_ALGO_DIRECT",
     "LQ_ALGO_DOUBLING", "LQ_PREC_FP64", "LQ_PREC_INT8X4", "LQ_PREC_INT8X5", "LQ_PREC_INT8X6", "LQ_PREC_INT8X7",
@@ -238,6 +238,20 @@ def spectral_moments(H: SparseMatrix, OP, phi: np.ndarray, M: int, algo: int = L
     mu = np.zeros(M, dtype=np.complex128) if mu is None else np.array(mu, dtype=np.complex128)
     _lib.call("lq_spectral_moments", H.h, _h(OP), pp, R, M, algo, _dp(mu.view(np.float64)))
     return mu
+
+
+def linear_grid(ctx: Context, kind: int, mu: np.ndarray, n: int, W: float, Ec: float, num_div: int | None = None):
+    """The linear-grid tools (``*FromChebmom.cpp``) on Jackson-damped moments:
+    kind LQ_LIN_*; num_div defaults to the tools' 30 * max moments.  Returns
+    (E, value) rows as the tool prints them."""
+    mu = np.ascontiguousarray(mu, dtype=np.complex128)
+    M0, M1 = (mu.shape[0], 1) if mu.ndim == 1 else mu.shape
+    if num_div is None:
+        num_div = 30 * max(M0, M1)
+    rows = num_div if kind == _lib.LQ_LIN_SPECTRAL else num_div - 1
+    E, out = np.zeros(num_div), np.zeros(num_div)
+    _lib.call("lq_linear_grid", ctx.h, kind, _dp(mu.view(np.float64)), M0, M1, n, W, Ec, num_div, _dp(E), _dp(out))
+    return E[:rows], out[:rows]
 
 
 class Batch:

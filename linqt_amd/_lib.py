@@ -25,6 +25,8 @@ LQ_KUBO_BASTIN = 1
 LQ_ALGO_DIRECT = 0
 LQ_ALGO_DOUBLING = 1
 LQ_PREC_FP64 = 0
+LQ_LIN_SPECTRAL, LQ_LIN_GREENWOOD, LQ_LIN_BASTIN, LQ_LIN_BASTIN_I, LQ_LIN_BASTIN_II, LQ_LIN_BASTIN_SEA, \
+    LQ_LIN_BASTIN_SURF = range(7)
 LQ_PREC_INT8X4 = 4
 LQ_PREC_INT8X5 = 5
 LQ_PREC_INT8X6 = 6
@@ -72,6 +74,8 @@ _SIGS = {
                                        _vp, C.c_int]),
     "lq_compute_mom_table_prec": (C.c_int, [_vp, _vp, C.c_longlong, C.c_int, _vp, C.c_longlong, C.c_int,
                                             C.c_longlong, C.c_int, _vp, C.c_int]),
+    "lq_linear_grid": (C.c_int, [_vp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                 _dp, _dp]),
     "lq_batch_create": (C.c_int, [_vp, C.c_size_t, C.c_int, C.POINTER(_vp)]),
     "lq_batch_destroy": (C.c_int, [_vp]),
     "lq_batch_job_bytes": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_size_t)]),

@@ -1387,4 +1387,30 @@ int lq_bastin_fftgrid(lq_ctx ctx, const double *mu, int M0, int M1, int n, doubl
   return grid2d(ctx, 1, mu, M0, M1, n, W, Ec, num_div, E, sigma);
 }
 
+int lq_linear_grid(lq_ctx ctx, int kind, const double *mu, int M0, int M1, int n, double W, double Ec, int num_div,
+                   double *E, double *out) {
+  if (!ctx || !mu || kind < LQ_LIN_SPECTRAL || kind > LQ_LIN_BASTIN_SURF || M0 <= 0 ||
+      (kind != LQ_LIN_SPECTRAL && M1 <= 0) || num_div <= 1 || !E || !out) {
+    set_error("lq_linear_grid: invalid arguments");
+    return LQ_ERR_ARG;
+  }
+  if (kind == LQ_LIN_SPECTRAL) M1 = 1;
+  DeviceGuard dg(ctx->device);
+  double2 *dm;
+  double *kern, *dout;
+  LQ_TRY(ctx_buf(ctx, "grid_mu", (size_t)M0 * M1 * sizeof(double2), (void **)&dm));
+  LQ_TRY(ctx_buf(ctx, "grid_scratch", (size_t)2 * num_div * sizeof(double), (void **)&kern));
+  LQ_TRY(ctx_buf(ctx, "grid_out", (size_t)2 * num_div * sizeof(double), (void **)&dout));
+  LQ_HIP_CHECK(hipMemcpyAsync(dm, mu, (size_t)M0 * M1 * sizeof(double2), hipMemcpyHostToDevice, ctx->stream));
+  hipEvent_t ev;
+  LQ_TRY(time_begin(ctx, kCatPost, &ev));
+  LQ_TRY(launch_linear_grid(ctx->stream, kind, dm, M0, M1, n, W, Ec, num_div, kern, dout, dout + num_div));
+  LQ_TRY(time_end(ctx, kCatPost, ev, 0));
+  const int rows = kind == LQ_LIN_SPECTRAL ? num_div : num_div - 1;
+  LQ_HIP_CHECK(hipMemcpyAsync(E, dout, rows * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LQ_HIP_CHECK(hipMemcpyAsync(out, dout + num_div, rows * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return collect_timing(ctx);
+}
+
 }  // extern "C"

@@ -183,6 +183,10 @@ int launch_greenwood_fftgrid(hipStream_t st, const double2 *mu, int M0, int M1, 
                              int num_div, double *scratch, double *E, double *out);
 int launch_bastin_fftgrid(hipStream_t st, const double2 *mu, int M0, int M1, int n, double W, double Ec,
                           int num_div, double *scratch, double *E, double *out);
+// Linear-grid tools (kind = LQ_LIN_*): kern scratch of num_div doubles; E, out
+// receive num_div (spectral) or num_div - 1 rows.
+int launch_linear_grid(hipStream_t st, int kind, const double2 *mu, int M0, int M1, int n, double W, double Ec,
+                       int num_div, double *kern, double *E, double *out);
 
 }  // namespace lq
 

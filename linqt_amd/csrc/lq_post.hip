@@ -284,6 +284,105 @@ __global__ __launch_bounds__(256) void k_grid2d(const double2 *__restrict__ mu, 
   if (threadIdx.x == 0) kern[i] = red[0];
 }
 
+// Linear-grid tools (src/*FromChebmom.cpp), kinds as LQ_LIN_*: one block per
+// energy evaluates the tool's double sum in separable form,
+//   sum_m1 proj( b_m1(x) * sum_m0 a_m0(x) mu(m0, m1) ),
+// with (a, b, proj) = (delta, gR, Im) Greenwood/BastinI, (delta, DgR, Im)
+// Bastin, (delta, delta, Re) Surf, (Im gR, Re DgR, 4 Im) Sea; BastinII uses
+// the two left vectors gR and DgR: -Re(DgR_m1 z_gR - gR_m1 z_DgR).
+__device__ __forceinline__ double lin_xgrid(int kind, int i, int num_div) {
+  const double xb = kind == 2 ? 0.99 : 1.0;
+  return -xb + i * (2 * xb) / (num_div - 1);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_linear_kernel(const double2 *__restrict__ mu, int M0, int M1, int num_div,
+                                                       double scale, double *__restrict__ kern) {
+  extern __shared__ double2 lv[];  // left vector(s): [M0] (BastinII: [2][M0])
+  const int i = blockIdx.x;
+  const double x = lin_xgrid(KIND, i, num_div);
+  for (int m0 = threadIdx.x; m0 < M0; m0 += blockDim.x) {
+    if (KIND == 4) {
+      lv[m0] = greenR_chebF(x, m0);
+      lv[M0 + m0] = DgreenR_chebF(x, m0);
+    } else if (KIND == 5) {
+      lv[m0] = make_double2(greenR_chebF(x, m0).y, 0.0);
+    } else {
+      lv[m0] = make_double2(delta_chebF(x, m0), 0.0);
+    }
+  }
+  __syncthreads();
+  double acc = 0;
+  if (KIND == 0) {
+    for (int m0 = threadIdx.x; m0 < M0; m0 += blockDim.x) acc += lv[m0].x * mu[m0].x;
+  } else {
+    for (int m1 = threadIdx.x; m1 < M1; m1 += blockDim.x) {
+      double zr = 0, zi = 0, wr = 0, wi = 0;
+      for (int m0 = 0; m0 < M0; ++m0) {
+        const double2 v = mu[(size_t)m0 * M1 + m1];
+        const double2 l = lv[m0];
+        if (KIND == 4) {
+          zr += l.x * v.x - l.y * v.y;
+          zi += l.x * v.y + l.y * v.x;
+          const double2 dl = lv[M0 + m0];
+          wr += dl.x * v.x - dl.y * v.y;
+          wi += dl.x * v.y + dl.y * v.x;
+        } else {
+          zr = fma(l.x, v.x, zr);
+          zi = fma(l.x, v.y, zi);
+        }
+      }
+      if (KIND == 1 || KIND == 3) {
+        const double2 g = greenR_chebF(x, m1);
+        acc += g.x * zi + g.y * zr;  // Im(g z)
+      } else if (KIND == 2) {
+        const double2 g = DgreenR_chebF(x, m1);
+        acc += g.x * zi + g.y * zr;
+      } else if (KIND == 6) {
+        acc += delta_chebF(x, m1) * zr;
+      } else if (KIND == 5) {
+        acc += 4.0 * DgreenR_chebF(x, m1).x * zi;
+      } else {  // 4: -Re(DgR z_gR - gR z_DgR)
+        const double2 d = DgreenR_chebF(x, m1), g = greenR_chebF(x, m1);
+        acc += -((d.x * zr - d.y * zi) - (g.x * wr - g.y * wi));
+      }
+    }
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) kern[i] = red[0] * scale;
+}
+
+// The printed columns, in the sources' left-to-right order (one thread: the
+// running sums must keep that order).
+__global__ void k_linear_finish(int kind, int num_div, double a, double Ec, const double *__restrict__ kern,
+                                double *__restrict__ E, double *__restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int rows = kind == 0 ? num_div : num_div - 1;
+  double acc = 0;
+  for (int i = 0; i < rows; ++i) {
+    const double x = lin_xgrid(kind, i, num_div);
+    E[i] = x * a + Ec;
+    if (kind == 0 || kind == 1) {
+      out[i] = kern[i];
+    } else {
+      const double dx = lin_xgrid(kind, i + 1, num_div) - x;
+      switch (kind) {
+        case 2: acc += kern[i] * dx; out[i] = acc; break;
+        case 3: out[i] = 0.5 * (kern[i] + kern[i + 1]); break;
+        case 4: acc += (kern[i] + kern[i + 1]) * dx; out[i] = acc; break;
+        case 5: acc += 0.5 * (kern[i] + kern[i + 1]) * dx; out[i] = acc; break;
+        default: out[i] = (kern[i + 1] + kern[i]) / 2.0; break;
+      }
+    }
+  }
+}
+
 __global__ void k_greenwood_energies(int num_div, double *__restrict__ e) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < num_div) e[i] = cos(kPi * (2 * i + 0.5) / num_div);
@@ -317,6 +416,36 @@ __global__ void k_bastin_finish(int num_div, int n, double W, double Ec, const d
 }
 
 }  // namespace
+
+int launch_linear_grid(hipStream_t st, int kind, const double2 *mu, int M0, int M1, int n, double W, double Ec,
+                       int num_div, double *kern, double *E, double *out) {
+  const double a = W / 2.0;
+  double scale;
+  switch (kind) {
+    case 0: scale = n / a; break;
+    case 1: case 3: scale = -1.0 * (n / a / a); break;
+    case 2: scale = -2.0 * n / a / a; break;
+    case 4: case 5: scale = n / a / a / 2 / kPi; break;
+    case 6: scale = kPi * n / a / a; break;
+    default:
+      set_error("launch_linear_grid: unknown kind %d", kind);
+      return LQ_ERR_ARG;
+  }
+  const size_t lds = sizeof(double2) * (size_t)M0 * (kind == 4 ? 2 : 1);
+  if (lds > 64 * 1024) {
+    set_error("launch_linear_grid: %d moments exceed the LDS table (max %d)", M0, kind == 4 ? 2048 : 4096);
+    return LQ_ERR_UNSUPPORTED;
+  }
+  switch (kind) {
+#define LQ_LIN(K) \
+  case K: hipLaunchKernelGGL((k_linear_kernel<K>), dim3(num_div), dim3(256), lds, st, mu, M0, M1, num_div, scale, kern); break;
+    LQ_LIN(0) LQ_LIN(1) LQ_LIN(2) LQ_LIN(3) LQ_LIN(4) LQ_LIN(5) LQ_LIN(6)
+#undef LQ_LIN
+  }
+  hipLaunchKernelGGL(k_linear_finish, dim3(1), dim3(64), 0, st, kind, num_div, a, Ec, kern, E, out);
+  LQ_HIP_CHECK(hipGetLastError());
+  return LQ_OK;
+}
 
 int launch_corr_epilogue(hipStream_t st, const double2 *raw, int ldr, const double2 *init, int M0, int M1, int B,
                          int R, double2 *out) {

@@ -52,6 +52,9 @@ def lib():
         L.lqo_kubo_post.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp]
         L.lqo_jackson_1d.argtypes = [_dp, C.c_int, C.c_double, C.c_double]
         L.lqo_jackson_2d.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _ip, _ip]
+        L.lqo_linear_kernel.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp]
+        L.lqo_linear_grid.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                      _dp, _dp]
         for fn in ("lqo_dos_fftgrid",):
             getattr(L, fn).argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
         for fn in ("lqo_greenwood_fftgrid", "lqo_bastin_fftgrid"):
@@ -207,6 +210,29 @@ def bastin_fftgrid(mu: np.ndarray, n: int, W: float, Ec: float, num_div: int):
     _check(lib().lqo_bastin_fftgrid(_d(mu), mu.shape[0], mu.shape[1], n, W, Ec, num_div, _d(E), _d(out)),
            "bastin_fftgrid")
     return E, out
+
+
+LINEAR_KINDS = {"spectral": 0, "greenwood": 1, "bastin": 2, "bastinI": 3, "bastinII": 4, "sea": 5, "surf": 6}
+
+
+def linear_kernel(kind: int, mu: np.ndarray, n: int, W: float, x: np.ndarray) -> np.ndarray:
+    """The linear-grid tool's kernel (scaled, before any running sum) at normalised energies x."""
+    mu = np.ascontiguousarray(mu, dtype=np.complex128)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    M0, M1 = (mu.shape[0], 1) if mu.ndim == 1 else mu.shape
+    k = np.zeros(len(x))
+    _check(lib().lqo_linear_kernel(kind, _d(mu), M0, M1, n, W, len(x), _d(x), _d(k)), "linear_kernel")
+    return k
+
+
+def linear_grid(kind: int, mu: np.ndarray, n: int, W: float, Ec: float, num_div: int):
+    """The whole tool: (E, value) rows as printed."""
+    mu = np.ascontiguousarray(mu, dtype=np.complex128)
+    M0, M1 = (mu.shape[0], 1) if mu.ndim == 1 else mu.shape
+    E, out = np.zeros(num_div), np.zeros(num_div)
+    rows = lib().lqo_linear_grid(kind, _d(mu), M0, M1, n, W, Ec, num_div, _d(E), _d(out))
+    _check(0 if rows > 0 else -1, "linear_grid")
+    return E[:rows], out[:rows]
 
 
 def omp_threads() -> int:

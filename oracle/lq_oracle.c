@@ -791,6 +791,100 @@ int lqo_bastin_fftgrid(const double *mu_, int M0, int M1, int n, double W, doubl
   return 0;
 }
 
+/* Kernels of the linear-grid tools, each in its source's loop order:
+ *   1 kuboGreenwoodFromChebmom.cpp:64-73, 3 kuboBastinIFromChebmom.cpp:66-74:
+ *     -N/a^2 sum delta_m0 Im(gR_m1 mu)
+ *   2 kuboBastinFromChebmom.cpp:64-72: -2N/a^2 sum delta_m0 Im(DgR_m1 mu)
+ *   4 kuboBastinIIFromChebmom.cpp:66-79:
+ *     N/a^2/(2 pi) sum -Re((gR_m0 DgR_m1 - DgR_m0 gR_m1) mu)
+ *   5 kuboBastinSeaFromChebmom.cpp:66-79:
+ *     N/a^2/(2 pi) sum -Re((Gr - conj Gr)(DGr + DGr) mu), Gr = gR_m0, DGr = Re DgR_m1
+ *   6 kuboBastinSurfFromChebmom.cpp:64-72: pi N/a^2 sum delta_m0 delta_m1 Re mu
+ *   0 spectralFunctionFromChebmom.cpp:60-67: N/a sum delta_m Re mu_m */
+int lqo_linear_kernel(int kind, const double *mu_, int M0, int M1, int n, double W, int nx, const double *x,
+                      double *kern) {
+  if (!mu_ || M0 <= 0 || (kind != 0 && M1 <= 0) || nx <= 0 || !x || !kern || kind < 0 || kind > 6) return -1;
+  const cplx *mu = (const cplx *)mu_;
+  const double a = W / 2.0;
+  for (int i = 0; i < nx; ++i) {
+    const double e = x[i];
+    double k = 0.0;
+    if (kind == 0) {
+      for (int m0 = 0; m0 < M0; ++m0) k += delta_chebF(e, m0) * creal(mu[m0]);
+      k *= n / a;
+    } else if (kind == 1 || kind == 3) {
+      double out = 0.0;
+      for (int m0 = 0; m0 < M0; ++m0)
+        for (int m1 = 0; m1 < M1; ++m1)
+          out += delta_chebF(e, m0) * cimag(greenR_chebF(e, m1) * mu[(size_t)m0 * M1 + m1]);
+      k = -1.0 * out * (n / a / a);
+    } else if (kind == 2) {
+      for (int m0 = 0; m0 < M0; ++m0)
+        for (int m1 = 0; m1 < M1; ++m1)
+          k += delta_chebF(e, m0) * cimag(DgreenR_chebF(e, m1) * mu[(size_t)m0 * M1 + m1]);
+      k *= -2.0 * n / a / a;
+    } else if (kind == 4) {
+      double out = 0.0;
+      for (int m0 = 0; m0 < M0; ++m0)
+        for (int m1 = 0; m1 < M1; ++m1) {
+          const cplx GrL = greenR_chebF(e, m0), GrR = greenR_chebF(e, m1);
+          const cplx DGrL = DgreenR_chebF(e, m0), DGrR = DgreenR_chebF(e, m1);
+          out += -creal((GrL * DGrR - DGrL * GrR) * mu[(size_t)m0 * M1 + m1]);
+        }
+      k = out * n / a / a / 2 / M_PI;
+    } else if (kind == 5) {
+      for (int m0 = 0; m0 < M0; ++m0)
+        for (int m1 = 0; m1 < M1; ++m1) {
+          const cplx Gr = greenR_chebF(e, m0), Ga = conj(greenR_chebF(e, m0));
+          const double DGr = creal(DgreenR_chebF(e, m1));
+          const cplx DGa = DGr - 0.0 * I;
+          k += -creal((Gr - Ga) * (DGr + DGa) * mu[(size_t)m0 * M1 + m1]);
+        }
+      k *= n / a / a / 2 / M_PI;
+    } else {
+      for (int m0 = 0; m0 < M0; ++m0)
+        for (int m1 = 0; m1 < M1; ++m1)
+          k += delta_chebF(e, m0) * delta_chebF(e, m1) * creal(mu[(size_t)m0 * M1 + m1]);
+      k *= M_PI * n / a / a;
+    }
+    kern[i] = k;
+  }
+  return 0;
+}
+
+/* The grids and the printed columns of the same tools (lines 35-41 and 76-86
+ * of each source; the Bastin tool uses xbound = 0.99). */
+int lqo_linear_grid(int kind, const double *mu, int M0, int M1, int n, double W, double Ec, int num_div,
+                    double *E_out, double *out) {
+  if (num_div <= 1 || !E_out || !out) return -1;
+  const double a = W / 2.0, xb = kind == 2 ? 0.99 : 1.0;
+  double *x = (double *)malloc(sizeof(double) * (size_t)num_div);
+  double *k = (double *)malloc(sizeof(double) * (size_t)num_div);
+  for (int i = 0; i < num_div; ++i) x[i] = -xb + i * (2 * xb) / (num_div - 1);
+  if (lqo_linear_kernel(kind, mu, M0, M1, n, W, num_div, x, k) != 0) {
+    free(x);
+    free(k);
+    return -1;
+  }
+  const int rows = kind == 0 ? num_div : num_div - 1;
+  double acc = 0;
+  for (int i = 0; i < rows; ++i) {
+    E_out[i] = x[i] * a + Ec;
+    switch (kind) {
+      case 0:
+      case 1: out[i] = k[i]; break;
+      case 2: acc += k[i] * (x[i + 1] - x[i]); out[i] = acc; break;
+      case 3: out[i] = 0.5 * (k[i] + k[i + 1]); break;
+      case 4: acc += (k[i] + k[i + 1]) * (x[i + 1] - x[i]); out[i] = acc; break;
+      case 5: acc += 0.5 * (k[i] + k[i + 1]) * (x[i + 1] - x[i]); out[i] = acc; break;
+      default: out[i] = (k[i + 1] + k[i]) / 2.0; break;
+    }
+  }
+  free(x);
+  free(k);
+  return rows;
+}
+
 /* ---- OpenMP CPU baseline --------------------------------------------------- */
 int lqo_omp_threads(void) {
 #ifdef _OPENMP

@@ -98,6 +98,20 @@ int lqo_greenwood_fftgrid(const double *mu, int M0, int M1, int n, double W,
 int lqo_bastin_fftgrid(const double *mu, int M0, int M1, int n, double W,
                        double Ec, int num_div, double *E_out, double *out);
 
+/* Linear-grid post-processors (src/...FromChebmom.cpp): kind 0 spectral
+ * function (1D mu of M0 moments), 1 Greenwood, 2 Bastin, 3 BastinI,
+ * 4 BastinII, 5 BastinSea, 6 BastinSurf (2D mu, M0 x M1).
+ * lqo_linear_kernel: the tool's kernel (with its scale factor, before any
+ * running sum) at nx normalised energies x.
+ * lqo_linear_grid: the whole tool on its grid of num_div energies
+ * x_i = -xb + i*2xb/(num_div-1) (xb = 0.99 for Bastin, 1 otherwise); writes
+ * the printed rows (num_div for kind 0, num_div-1 otherwise) and returns
+ * their count, or -1. */
+int lqo_linear_kernel(int kind, const double *mu, int M0, int M1, int n, double W, int nx,
+                      const double *x, double *kern);
+int lqo_linear_grid(int kind, const double *mu, int M0, int M1, int n, double W, double Ec,
+                    int num_div, double *E_out, double *out);
+
 /* ---- OpenMP CPU baseline (same algorithm, threads over rows/pairs) ----- */
 int  lqo_omp_threads(void);
 /* One Chebyshev step on a completed operator: y <- 2*H~*x - y for k vectors

@@ -34,7 +34,9 @@ def test_drivers_print_usage_without_gpu():
     """The binaries load and link (no GPU call happens on the usage path)."""
     for exe in ("inline_compute-kpm-spectralOp", "inline_compute-kpm-nonEqOp", "inline_compute-Kubo-kpm-FFT",
                 "spectralFunctionFromChebmom_FFTgrid", "kuboGreenwoodFromChebmom_FFTgrid",
-                "kuboBastinFromChebmom_FFTgrid"):
+                "kuboBastinFromChebmom_FFTgrid", "spectralFunctionFromChebmom", "kuboGreenwoodFromChebmom",
+                "kuboBastinFromChebmom", "kuboBastinIFromChebmom", "kuboBastinIIFromChebmom",
+                "kuboBastinSeaFromChebmom", "kuboBastinSurfFromChebmom"):
         r = subprocess.run([os.path.join(BIN, exe)], capture_output=True, text=True, timeout=60)
         assert r.returncode == 0 and "should be called" in r.stdout, exe
 
@@ -121,3 +123,44 @@ def test_kubo_fft_driver_matches_oracle(chain_dir, oracle, formula):
     Eo, so, _ = oracle.kubo_post(fid, fd, H.n, 10.0, 0.0)
     assert np.allclose(E, Eo, rtol=1e-5, atol=1e-9)
     assert rel_inf(s, so) < 1e-5
+
+
+def _rows_with_nan(path):
+    rows = [line.split() for line in open(path) if line.strip()]
+    return np.array([float(r[0]) for r in rows]), np.array([float(r[1]) for r in rows])
+
+
+@pytest.mark.gpu
+def test_linear_grid_drivers_match_oracle(chain_dir, oracle):
+    """The seven linear-grid programs on the golden linear-chain moments: same file names,
+    30*M rows on the tools' grids, values to the 6 digits of the format (NaN where the
+    reference's formulas divide by zero at x = +-1)."""
+    g = os.path.join(LC, "NonLEqOpVX-VXlinear_chainKPM_M100x100.chebmom2D")
+    shutil.copy(g, chain_dir / "g.chebmom2D")
+    mu, n, W, Ec = io.read_chebmom2d(g)
+    muJ = oracle.jackson_2d(mu, W, 80.0, 80.0)
+    nd = 30 * max(muJ.shape)
+    for exe, prefix, kind in (("kuboGreenwoodFromChebmom", "KuboGreenWood_", 1),
+                              ("kuboBastinFromChebmom", "KuboBastin_", 2),
+                              ("kuboBastinIFromChebmom", "KuboBastinI_", 3),
+                              ("kuboBastinIIFromChebmom", "KuboBastinII_", 4),
+                              ("kuboBastinSeaFromChebmom", "KuboBastinSea_", 5),
+                              ("kuboBastinSurfFromChebmom", "KuboBastinSurf_", 6)):
+        _run(exe, ["g.chebmom2D", 80], chain_dir)
+        E, v = _rows_with_nan(chain_dir / f"{prefix}gJACKSON.dat")
+        Eo, vo = oracle.linear_grid(kind, muJ, n, W, Ec, nd)
+        assert len(E) == len(Eo) == nd - 1, exe
+        assert np.allclose(E, Eo, rtol=1e-5, atol=1e-9), exe
+        assert np.array_equal(np.isfinite(v), np.isfinite(vo)), exe
+        f = np.isfinite(vo)
+        if f.any():
+            assert np.abs(v[f] - vo[f]).max() <= 2e-5 * np.abs(vo[f]).max(), exe
+    # the 1D tool on a 1D moment file made from the table's first column
+    io.write_chebmom1d(str(chain_dir / "s.chebmom1D"), mu[:, 0], n, W, Ec)
+    _run("spectralFunctionFromChebmom", ["s.chebmom1D", 80], chain_dir)
+    E, v = _rows_with_nan(chain_dir / "meansJACKSON.dat")
+    m1 = oracle.jackson_1d(io.read_chebmom1d(str(chain_dir / "s.chebmom1D"))[0], W, 80.0)
+    Eo, vo = oracle.linear_grid(0, m1, n, W, Ec, 30 * len(m1))
+    assert len(E) == len(Eo) and np.array_equal(np.isfinite(v), np.isfinite(vo))
+    f = np.isfinite(vo)
+    assert np.abs(v[f] - vo[f]).max() <= 2e-5 * np.abs(vo[f]).max()

@@ -276,3 +276,32 @@ def test_fused_right_operators_match_separate_applies(ctx, oracle, monkeypatch):
     got = lq.correlation_expansion_moments(G[0], G[1], [G[2], X], phi, 16, 16)
     assert rel_inf(got[1], oracle.correlation_moments(O[0], O[1], OX, phi, 16, 16)) < 1e-10
     assert rel_inf(got[0], oracle.correlation_moments(O[0], O[1], O[2], phi, 16, 16)) < 1e-10
+
+
+def _same_rows(got, ref, tol):
+    gE, gv = got
+    rE, rv = ref
+    assert len(gE) == len(rE) and np.allclose(gE, rE, rtol=0, atol=1e-13 * max(1.0, np.abs(rE).max()))
+    assert np.array_equal(np.isfinite(gv), np.isfinite(rv))  # +-1: NaN or inf by summation order
+    f = np.isfinite(rv)
+    if f.any():
+        assert np.abs(gv[f] - rv[f]).max() <= tol * np.abs(rv[f]).max()
+
+
+@pytest.mark.parametrize("kind", range(7))
+def test_linear_grid_tools_vs_oracle(ctx, oracle, kind):
+    say("linear grid", kind)
+    rng = np.random.default_rng(kind)
+    mu = rng.normal(size=(36, 28)) + 1j * rng.normal(size=(36, 28))
+    if kind == 0:
+        mu = mu[:, 0].copy()
+    n, W, Ec = 123, 7.5, -0.3
+    nd = 30 * (36 if kind else 36)
+    _same_rows(lq.linear_grid(ctx, kind, mu, n, W, Ec, nd), oracle.linear_grid(kind, mu, n, W, Ec, nd), 1e-10)
+    # golden linear-chain table, Jackson-damped as the tools do
+    g, gn, gW, gEc = io.read_chebmom2d(os.path.join(LC, "NonLEqOpVX-VXlinear_chainKPM_M100x100.chebmom2D"))
+    muJ = oracle.jackson_2d(g, gW, 60.0, 60.0)
+    if kind == 0:
+        muJ = muJ[:, 0].copy()
+    nd = 30 * max(np.atleast_2d(muJ).shape)
+    _same_rows(lq.linear_grid(ctx, kind, muJ, gn, gW, gEc), oracle.linear_grid(kind, muJ, gn, gW, gEc, nd), 1e-9)

@@ -324,3 +324,103 @@ def test_bastin_fftgrid_trig_form(oracle):
         num = (g(xx + h, mm) - g(xx - h, mm)) / (2 * h)
         ana = -1j * np.exp(-1j * mm * np.arccos(xx)) * (xx + 1j * mm * np.sqrt(1 - xx * xx)) / (1 - xx * xx) ** 1.5
         assert abs(num - ana) < 1e-6 * max(1, abs(ana))
+
+
+# ---- linear-grid tools (src/*FromChebmom.cpp) --------------------------------------
+
+
+def _unit(M0, M1, m0, m1, v):
+    mu = np.zeros((M0, M1), dtype=np.complex128)
+    mu[m0, m1] = v
+    return mu
+
+
+def test_linear_coefficients_match_closed_forms(oracle):
+    """delta_m = T_m(x)/(pi sqrt(1-x^2)) (scipy's Chebyshev T), gR_m = -i e^{-im theta}/sin theta and
+    DgR_m = -i e^{-im theta}(cos theta + i m sin theta)/sin^3 theta (x = cos theta), read back through
+    single-moment tables: kind 0 gives delta, kinds 1 and 2 give Im and Re of gR / DgR times delta_0."""
+    from scipy.special import eval_chebyt
+
+    x = np.linspace(-0.97, 0.95, 37)
+    th, f0 = np.arccos(x), np.sqrt(1 - x * x)
+    n, W = 7, 4.0
+    a = W / 2
+    for m in (0, 1, 5, 17):
+        k = oracle.linear_kernel(0, _unit(m + 1, 1, m, 0, 1.0)[:, 0], n, W, x)
+        ref = eval_chebyt(m, x) / (np.pi * f0) * n / a
+        assert np.allclose(k, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+        d0 = 1 / (np.pi * f0)
+        gR = -1j * np.exp(-1j * m * th) / f0
+        DgR = -1j * np.exp(-1j * m * th) * (x + 1j * m * f0) / f0 ** 3
+        for kind, g, s in ((1, gR, -n / a / a), (2, DgR, -2 * n / a / a)):
+            for v in (1.0, 1j):  # Im(g v): v = 1 -> Im g, v = i -> Re g
+                k = oracle.linear_kernel(kind, _unit(1, m + 1, 0, m, v), n, W, x)
+                ref = s * d0 * np.imag(g * v)
+                assert np.allclose(k, ref, rtol=1e-10, atol=1e-12 * np.abs(s * d0 * g).max())
+
+
+def _jackson_table(oracle, M0=40, M1=32, seed=2):
+    rng = np.random.default_rng(seed)
+    mu = rng.normal(size=(M0, M1)) + 1j * rng.normal(size=(M0, M1))
+    return mu
+
+
+def test_linear_greenwood_and_bastin_match_fftgrid_forms(oracle):
+    """The linear tools' Greenwood and Bastin kernels at the FFT-grid tools' energies equal
+    those (independently pinned) tools; the spectral kernel's magnitude equals the DOS grid."""
+    mu = _jackson_table(oracle)
+    n, W, Ec, nd = 50, 6.0, 0.2, 64
+    a = W / 2
+    xg = np.cos(np.pi * (2 * np.arange(nd) + 0.5) / nd)[: nd - 1]
+    Eg, sg = oracle.greenwood_fftgrid(mu, n, W, Ec, nd)
+    assert rel_inf(oracle.linear_kernel(1, mu, n, W, xg), sg) < 1e-12
+    Eb, sb = oracle.bastin_fftgrid(mu, n, W, Ec, nd)
+    xb = (Eb - Ec) / a
+    inc = np.diff(np.concatenate([[0.0], sb]))           # k_i (x_{i+1} - x_i)
+    xs = np.sort(np.cos(np.pi * (2 * np.arange(nd) + 0.5) / nd))
+    kb = oracle.linear_kernel(2, mu, n, W, xb)
+    assert rel_inf(kb * np.diff(xs), inc) < 1e-10
+    mu1 = mu[:, 0]
+    Ed, rd = oracle.dos_fftgrid(mu1, n, W, Ec, nd)
+    xd = (Ed - Ec) / a
+    assert rel_inf(np.abs(oracle.linear_kernel(0, mu1, n, W, xd)), rd) < 1e-12
+
+
+def test_linear_surf_sea_bastinII_identities(oracle):
+    """Im gR_m = -pi delta_m gives: Surf = Greenwood for a real table; Sea = Bastin for an
+    imaginary table; the BastinII integrand is antisymmetric in (m0, m1), so 0 for a symmetric table."""
+    rng = np.random.default_rng(4)
+    M = 24
+    x = np.linspace(-0.9, 0.93, 41)
+    n, W = 11, 5.0
+    R = rng.normal(size=(M, M))
+    assert rel_inf(oracle.linear_kernel(6, R + 0j, n, W, x), oracle.linear_kernel(1, R + 0j, n, W, x)) < 1e-11
+    assert rel_inf(oracle.linear_kernel(5, 1j * R, n, W, x), oracle.linear_kernel(2, 1j * R, n, W, x)) < 1e-11
+    S = R + R.T + 1j * (R - R.T) * 0
+    k4 = oracle.linear_kernel(4, S + 0j, n, W, x)
+    scale = np.abs(oracle.linear_kernel(2, S + 0j, n, W, x)).max()
+    assert np.abs(k4).max() < 1e-10 * scale
+    assert np.abs(oracle.linear_kernel(4, R + 0j, n, W, x)).max() > 1e-3 * scale
+
+
+def test_linear_grid_rows_and_end_points(oracle):
+    """Grids and printed columns: 30 M energies, xb = 0.99 for Bastin; NaN at x = +-1 (the
+    sources divide by sqrt(1 - x^2) there) and NaN-filled running sums for BastinII/Sea."""
+    mu = _jackson_table(oracle, 10, 8)
+    n, W, Ec = 20, 4.0, 0.1
+    nd = 300
+    for kind in range(1, 7):
+        E, o = oracle.linear_grid(kind, mu, n, W, Ec, nd)
+        assert len(E) == nd - 1
+        xb = 0.99 if kind == 2 else 1.0
+        assert np.allclose(E, (-xb + np.arange(nd - 1) * 2 * xb / (nd - 1)) * W / 2 + Ec, rtol=0, atol=1e-14)
+        if kind == 2:
+            assert np.all(np.isfinite(o))
+            k = oracle.linear_kernel(2, mu, n, W, (E - Ec) / (W / 2))
+            assert rel_inf(np.diff(np.concatenate([[0], o])), k * (2 * xb / (nd - 1))) < 1e-9
+        elif kind in (4, 5):
+            assert np.all(np.isnan(o))
+        else:
+            assert np.isnan(o[0]) and np.all(np.isfinite(o[1:-1]))
+    E, o = oracle.linear_grid(0, mu[:, 0], n, W, Ec, nd)
+    assert len(E) == nd and np.isnan(o[0]) and np.isnan(o[-1]) and np.all(np.isfinite(o[1:-1]))

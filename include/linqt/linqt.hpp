@@ -330,6 +330,38 @@ class Moments2D : public Moments {
   std::array<int, 2> numMoms;
 };
 
+// Time-dependent moments mu(m, n), m < numMoms, n < maxTimeStep (row m stride
+// maxTimeStep), include/chebyshev_moments.hpp:287-365.
+class MomentsTD : public Moments {
+ public:
+  const double HBAR = 0.6582119624;  // eV fs
+  MomentsTD() : _numMoms(1), _maxTimeStep(1), _timeStep(0), _dt(0) {}
+  MomentsTD(const size_t m, const size_t n) : _numMoms(m), _maxTimeStep(n), _timeStep(0), _dt(0) {
+    MomentVector(vector_t(m * n, 0.0));
+  }
+  MomentsTD(std::string momfilename);
+  size_t MomentNumber() const { return _numMoms; }
+  size_t HighestMomentNumber() const { return _numMoms; }
+  size_t CurrentTimeStep() const { return _timeStep; }
+  size_t MaxTimeStep() const { return _maxTimeStep; }
+  double TimeDiff() const { return _dt; }
+  double ChebyshevFreq() const { return HalfWidth() / CUTOFF / HBAR; }
+  double ChebyshevFreq_0() const { return BandCenter() / HBAR; }
+  void MomentNumber(const size_t mom);
+  void MaxTimeStep(const size_t maxTimeStep) { _maxTimeStep = maxTimeStep; }
+  void IncreaseTimeStep() { _timeStep++; }
+  void ResetTime() { _timeStep = 0; }
+  void TimeDiff(const double dt) { _dt = dt; }
+  value_t &operator()(const size_t m, const size_t n) { return MomentVector((int)(m * MaxTimeStep() + n)); }
+  void ApplyJacksonKernel(const double broad);
+  void saveIn(std::string filename);
+  void Print();
+
+ private:
+  size_t _numMoms, _maxTimeStep, _timeStep;
+  double _dt;
+};
+
 // Stored Chebyshev vectors (host lists; the fused solvers never materialise them).
 class Vectors_sliced : public Moments {
  public:
@@ -424,6 +456,8 @@ int CorrelationExpansionMoments(const vector_t &PhiR, const vector_t &PhiL, Spar
 int CorrelationExpansionMoments(SparseMatrixType &OPL, SparseMatrixType &OPR, chebyshev::Moments2D &chebMoms,
                                 qstates::generator &gen);
 int SpectralMoments(SparseMatrixType &OP, chebyshev::Moments1D &chebMoms, qstates::generator &gen);
+int TimeDependentCorrelations(SparseMatrixType &OPL, SparseMatrixType &OPR, chebyshev::MomentsTD &chebMoms,
+                              qstates::generator &gen);
 
 // ---------------------------------------------------------------------------
 // include/Kubo_solver_FFT.hpp

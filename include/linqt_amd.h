@@ -180,6 +180,23 @@ int lq_greenwood_fftgrid(lq_ctx ctx, const double *mu, int M0, int M1, int n, do
 int lq_bastin_fftgrid(lq_ctx ctx, const double *mu, int M0, int M1, int n, double W, double Ec,
                       int num_div, double *E, double *sigma);
 
+/* ---- time-dependent correlations ------------------------------------------ */
+/* Replaces chebyshev::TimeDependentCorrelations (src/chebyshev_solver.cpp:231-275)
+ * with MomentsTD::Evolve (src/chebyshev_momentsTD.cpp:4-45): mu (M x nT complex,
+ * mu(m, n) at m*nT + n) += (2/R)(1/2)^{delta_m0} <OPL phi(t_n)|T_m(H~) OPR|phi(t_n)>,
+ * phi(t) = exp(-iHt/hbar) phi evolved in steps dt (fs) by the Bessel-Chebyshev
+ * expansion (terms while |J_k| > 1e-10).  H already rescaled with bandwidth W
+ * and centre Ec (eV), which set the expansion's frequencies.  OPL/OPR NULL = 1. */
+int lq_time_correlations(lq_csr H, lq_csr OPL, lq_csr OPR, const double *phi, int R, int M, int nT,
+                         double dt, double W, double Ec, double *mu);
+/* MomentsTD::ApplyJacksonKernel (src/chebyshev_momentsTD.cpp:135-154): the first
+ * *M_out rows of the M x nT table are kept and damped in place. */
+int lq_jackson_td(lq_ctx ctx, double *mu, int M, int nT, double W, double broad_meV, int *M_out);
+/* timeCorrelationsFromChebmom.cpp:35-70: nT rows (n*dt, N/a sum_m delta_m(x_F) Re mu(m, n)),
+ * x_F = (EF - Ec)/a. */
+int lq_timecorr_post(lq_ctx ctx, const double *mu, int M, int nT, int n, double W, double Ec, double dt,
+                     double EF, double *t, double *out);
+
 /* Linear-grid post-processors: the tools' grid is num_div energies
  * x_i = -xb + i*2xb/(num_div-1) (xb = 0.99 for Bastin, 1 otherwise; the tools
  * use num_div = 30 * max moments), E = a*x + Ec.  Rows written: num_div for

@@ -32,7 +32,8 @@ from ._lib import (LQ_ALGO_DIRECT, LQ_ALGO_DOUBLING, LQ_FILL_GENERAL, LQ_FILL_HE
                    LQ_PREC_INT8X6, LQ_PREC_INT8X7, LinqtError)
 
 __all__ = [
-    "Context", "SparseMatrix", "DeviceStates", "Batch", "linear_grid", "spectral_moments", "correlation_expansion_moments", "kubo_fft", "kubo_post",
+    "Context", "SparseMatrix", "DeviceStates", "Batch", "linear_grid", "time_correlations", "jackson_td",
+    "timecorr_post", "spectral_moments", "correlation_expansion_moments", "kubo_fft", "kubo_post",
     "jackson_1d", "jackson_2d", "dos_fftgrid", "greenwood_fftgrid", "bastin_fftgrid", "LinqtError",
     "LQ_FILL_HERMITIAN_UPPER", "LQ_FILL_GENERAL", "LQ_KUBO_GREENWOOD", "LQ_KUBO_BASTIN", "LQ_ALGO_DIRECT",
     "LQ_ALGO_DOUBLING", "LQ_PREC_FP64", "LQ_PREC_INT8X4", "LQ_PREC_INT8X5", "LQ_PREC_INT8X6", "LQ_PREC_INT8X7",
@@ -238,6 +239,31 @@ def spectral_moments(H: SparseMatrix, OP, phi: np.ndarray, M: int, algo: int = L
     mu = np.zeros(M, dtype=np.complex128) if mu is None else np.array(mu, dtype=np.complex128)
     _lib.call("lq_spectral_moments", H.h, _h(OP), pp, R, M, algo, _dp(mu.view(np.float64)))
     return mu
+
+
+def time_correlations(H: SparseMatrix, OPL, OPR, phi, M: int, nT: int, dt: float, W: float, Ec: float,
+                      mu=None) -> np.ndarray:
+    """``chebyshev::TimeDependentCorrelations``: (M, nT) table mu(m, n) at t_n = n dt (fs),
+    accumulated into mu when given.  H rescaled with bandwidth W and centre Ec."""
+    pp, R, keep = _states(phi)
+    out = np.zeros((M, nT), dtype=np.complex128) if mu is None else np.array(mu, dtype=np.complex128)
+    _lib.call("lq_time_correlations", H.h, _h(OPL), _h(OPR), pp, R, M, nT, dt, W, Ec, _dp(out.view(np.float64)))
+    return out
+
+
+def jackson_td(ctx: Context, mu: np.ndarray, W: float, broad: float) -> np.ndarray:
+    mu = np.array(mu, dtype=np.complex128, order="C")
+    mo = C.c_int(0)
+    _lib.call("lq_jackson_td", ctx.h, _dp(mu.view(np.float64)), mu.shape[0], mu.shape[1], W, broad, C.byref(mo))
+    return mu[: mo.value].copy()
+
+
+def timecorr_post(ctx: Context, mu: np.ndarray, n: int, W: float, Ec: float, dt: float, EF: float):
+    mu = np.ascontiguousarray(mu, dtype=np.complex128)
+    t, o = np.zeros(mu.shape[1]), np.zeros(mu.shape[1])
+    _lib.call("lq_timecorr_post", ctx.h, _dp(mu.view(np.float64)), mu.shape[0], mu.shape[1], n, W, Ec, dt, EF,
+              _dp(t), _dp(o))
+    return t, o
 
 
 def linear_grid(ctx: Context, kind: int, mu: np.ndarray, n: int, W: float, Ec: float, num_div: int | None = None):

@@ -74,6 +74,11 @@ _SIGS = {
                                        _vp, C.c_int]),
     "lq_compute_mom_table_prec": (C.c_int, [_vp, _vp, C.c_longlong, C.c_int, _vp, C.c_longlong, C.c_int,
                                             C.c_longlong, C.c_int, _vp, C.c_int]),
+    "lq_time_correlations": (C.c_int, [_vp, _vp, _vp, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                       C.c_double, _dp]),
+    "lq_jackson_td": (C.c_int, [_vp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, _ip]),
+    "lq_timecorr_post": (C.c_int, [_vp, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, _dp, _dp]),
     "lq_linear_grid": (C.c_int, [_vp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                  _dp, _dp]),
     "lq_batch_create": (C.c_int, [_vp, C.c_size_t, C.c_int, C.POINTER(_vp)]),

@@ -464,6 +464,79 @@ void Moments2D::Print() {
             << std::endl;
 }
 
+// ---- MomentsTD (src/chebyshev_momentsTD.cpp) --------------------------------------------
+MomentsTD::MomentsTD(std::string momfilename) : _numMoms(1), _maxTimeStep(1), _timeStep(0), _dt(0) {
+  const std::size_t ext = momfilename.find(".chebmomTD");
+  if (ext == std::string::npos) {
+    std::cerr << "The first argument does not seem to be a valid .chebmomTD file" << std::endl;
+    assert(false);
+    std::abort();
+  }
+  SystemLabel(momfilename.substr(0, ext));
+  std::ifstream f(momfilename.c_str());
+  assert(f.is_open());
+  int ib;
+  double db;
+  f >> ib;
+  SystemSize(ib);
+  f >> db;
+  BandWidth(db);
+  f >> db;
+  BandCenter(db);
+  f >> ib;
+  MaxTimeStep(ib);
+  f >> db;
+  TimeDiff(db);
+  f >> _numMoms >> _maxTimeStep;
+  MomentVector(vector_t(_numMoms * _maxTimeStep, 0.0));
+  for (size_t m = 0; m < _numMoms; m++)
+    for (size_t n = 0; n < _maxTimeStep; n++) {
+      double re, im;
+      f >> re >> im;
+      (*this)(m, n) = value_t(re, im);
+    }
+}
+
+void MomentsTD::MomentNumber(const size_t numMoms) {
+  assert(numMoms <= HighestMomentNumber());
+  vector_t nm(numMoms * _maxTimeStep, 0.0);
+  for (size_t m = 0; m < numMoms; m++)
+    for (size_t n = 0; n < _maxTimeStep; n++) nm[m * _maxTimeStep + n] = (*this)(m, n);
+  _numMoms = numMoms;
+  MomentVector(nm);
+}
+
+void MomentsTD::ApplyJacksonKernel(const double broad) {
+  int mo = 0;
+  vector_t &mu = MomentVector();
+  check(lq_jackson_td(default_context(), reinterpret_cast<double *>(mu.data()), (int)_numMoms, (int)_maxTimeStep,
+                      BandWidth(), broad, &mo),
+        "lq_jackson_td");
+  std::cout << "Kernel reduced the number of moments to " << mo << " for a broadening of " << M_PI / mo << std::endl;
+  mu.resize((size_t)mo * _maxTimeStep);
+  _numMoms = (size_t)mo;
+}
+
+void MomentsTD::saveIn(std::string filename) {
+  std::ofstream out(filename.c_str());
+  out.precision(std::numeric_limits<double>::digits10);
+  out << SystemSize() << " " << BandWidth() << " " << BandCenter() << " " << MaxTimeStep() << " " << TimeDiff() << " "
+      << std::endl;
+  out << HighestMomentNumber() << " " << MaxTimeStep() << " " << std::endl;
+  for (auto m : MomentVector()) out << m.real() << " " << m.imag() << std::endl;
+}
+
+void MomentsTD::Print() {
+  std::cout << "\n\nCHEBYSHEV TD MOMENTS INFO" << std::endl;
+  std::cout << "\tSYSTEM:\t\t\t" << SystemLabel() << std::endl;
+  if (SystemSize() > 0) std::cout << "\tSIZE:\t\t\t" << SystemSize() << std::endl;
+  std::cout << "\tMOMENTS SIZE:\t\t(" << HighestMomentNumber() << " x " << MaxTimeStep() << ")" << std::endl;
+  std::cout << "\tSCALE FACTOR:\t\t" << ScaleFactor() << std::endl;
+  std::cout << "\tSHIFT FACTOR:\t\t" << ShiftFactor() << std::endl;
+  std::cout << "\tTIME STEP:\t\t" << MaxTimeStep() << std::endl;
+  std::cout << "\tTIME DIFF:\t" << TimeDiff() << std::endl;
+}
+
 // ---- Vectors_sliced (src/chebyshev_vectors.cpp) --------------------------------------
 int Vectors_sliced::CreateVectorSet() {
   SystemSize(Hamiltonian().rank());
@@ -590,6 +663,22 @@ int CorrelationExpansionMoments(SparseMatrixType &OPL, SparseMatrixType &OPR, Mo
                                reinterpret_cast<const double *>(phi.data()), R, M0, M1, batchSize, LQ_PREC_FP64,
                                reinterpret_cast<double *>(chebMoms.MomentVector().data())),
         "lq_correlation_moments");
+  return 0;
+}
+
+// TimeDependentCorrelations (src/chebyshev_solver.cpp:231-275): one device solve over
+// all states and time steps.
+int TimeDependentCorrelations(SparseMatrixType &OPL, SparseMatrixType &OPR, MomentsTD &chebMoms,
+                              qstates::generator &gen) {
+  int R = 0;
+  std::vector<cplx> phi = collect_states(gen, (int)chebMoms.SystemSize(), &R);
+  if (R == 0) return 0;
+  check(lq_time_correlations(chebMoms.Hamiltonian().handle(), OPL.isIdentity() ? nullptr : OPL.handle(),
+                             OPR.isIdentity() ? nullptr : OPR.handle(), reinterpret_cast<const double *>(phi.data()),
+                             R, (int)chebMoms.HighestMomentNumber(), (int)chebMoms.MaxTimeStep(), chebMoms.TimeDiff(),
+                             chebMoms.BandWidth(), chebMoms.BandCenter(),
+                             reinterpret_cast<double *>(chebMoms.MomentVector().data())),
+        "lq_time_correlations");
   return 0;
 }
 

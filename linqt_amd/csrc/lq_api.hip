@@ -1268,6 +1268,170 @@ int lq_kubo_fft(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, const double 
   return collect_timing(ctx);
 }
 
+// ---- time-dependent correlations ------------------------------------------------
+// chebyshev::TimeDependentCorrelations (src/chebyshev_solver.cpp:231-275) with
+// MomentsTD::Evolve (src/chebyshev_momentsTD.cpp:4-45).  States run in groups;
+// the left and right vectors of a group evolve as the two sets of one SpMM per
+// Bessel term.
+int lq_time_correlations(lq_csr H, lq_csr OPL, lq_csr OPR, const double *phi, int R, int M, int nT, double dt,
+                         double W, double Ec, double *mu) {
+  LQ_TRY(check_op(H, "lq_time_correlations"));
+  if (!phi || R <= 0 || M <= 0 || nT <= 0 || !mu || !(W > 0) || (OPL && (OPL->n != H->n || OPL->ctx != H->ctx)) ||
+      (OPR && (OPR->n != H->n || OPR->ctx != H->ctx))) {
+    set_error("lq_time_correlations: invalid arguments");
+    return LQ_ERR_ARG;
+  }
+  lq_ctx ctx = H->ctx;
+  DeviceGuard dg(ctx->device);
+  const int n = H->n;
+  // Evolve's coefficients: c_0 = J_0(x), c_k = 2 (-i)^k J_k(x) while |J_k| > 1e-10,
+  // x = dt a / hbar (the reference's besselJ is libm's j0 / j1 / jn)
+  const double kHbar = 0.6582119624, a = W / 2.0, x = dt * (a / kHbar), tol = 1e-10;
+  std::vector<std::complex<double>> coef;
+  {
+    std::complex<double> nIp(1.0, 0.0);
+    double Jn = 0.5 * j0(x), cut = 2.0 * tol;
+    int k = 0;
+    while (cut > tol) {
+      coef.push_back(nIp * 2.0 * Jn);
+      nIp *= std::complex<double>(0.0, -1.0);
+      ++k;
+      Jn = k == 1 ? j1(x) : jn(k, x);
+      cut = fabs(Jn);
+    }
+  }
+  const std::complex<double> e0 = std::exp(std::complex<double>(0.0, -(Ec / kHbar) * dt));
+  const int g = std::min(R, 32);
+  std::vector<std::complex<double>> acc((size_t)M * nT, 0.0);
+  const double2 one = d2(1, 0), zero = d2(0, 0), two = d2(2, 0), mone = d2(-1, 0);
+  for (int r0 = 0; r0 < R; r0 += g) {
+    const int gg = std::min(g, R - r0);
+    int kp = 1;
+    while (kp < gg) kp <<= 1;
+    const size_t vec = (size_t)n * gg;
+    const int grid = spmm_dot_grid(n, kp);
+    double2 *pr, *pl, *c0, *c1, *d0, *d1, *part, *sums;
+    LQ_TRY(ctx_buf(ctx, "td_phiR", vec * sizeof(double2), (void **)&pr));
+    LQ_TRY(ctx_buf(ctx, "td_phiL", vec * sizeof(double2), (void **)&pl));
+    LQ_TRY(ctx_buf(ctx, "td_c0", 2 * vec * sizeof(double2), (void **)&c0));
+    LQ_TRY(ctx_buf(ctx, "td_c1", 2 * vec * sizeof(double2), (void **)&c1));
+    d0 = c0 + vec;  // right-vector halves of the evolution pair
+    d1 = c1 + vec;
+    LQ_TRY(ctx_buf(ctx, "td_part", (size_t)M * grid * kp * sizeof(double2), (void **)&part));
+    LQ_TRY(ctx_buf(ctx, "td_sums", (size_t)M * kp * sizeof(double2), (void **)&sums));
+    LQ_TRY(upload_states(ctx, phi, n, r0, gg, pr));
+    if (OPL) {
+      VecSet s{pr, nullptr, pl, gg};
+      LQ_TRY(spmm(ctx, OPL, one, zero, &s, 1));
+    } else {
+      LQ_TRY(launch_copy(ctx->stream, pl, pr, vec));
+    }
+    const size_t slot = (size_t)grid * kp;
+    std::vector<double2> h((size_t)M * kp);
+    for (int t = 0; t < nT; ++t) {
+      // moments at t: T_0 = OPR phi_R, mu(m, t) += scal <phi_L|T_m>
+      double2 *A = c0, *B = c1;
+      if (OPR) {
+        VecSet s{pr, nullptr, A, gg};
+        LQ_TRY(spmm(ctx, OPR, one, zero, &s, 1));
+      } else {
+        LQ_TRY(launch_copy(ctx->stream, A, pr, vec));
+      }
+      LQ_TRY(launch_dot(ctx->stream, n, gg, pl, A, part, nullptr));
+      {
+        SpmvLoopGuard guard{ctx};
+        LQ_TRY(spmv_loop_begin(ctx));
+        if (M > 1) {
+          VecSet s{A, nullptr, B, gg};
+          LQ_TRY(spmm(ctx, H, one, zero, &s, 1, kDotPhi, pl, part + slot));
+        }
+        for (int m = 1; m + 1 < M; ++m) {
+          VecSet s{B, A, A, gg};
+          LQ_TRY(spmm(ctx, H, two, mone, &s, 1, kDotPhi, pl, part + (size_t)(m + 1) * slot));
+          std::swap(A, B);
+        }
+        LQ_TRY(spmv_loop_end(ctx));
+      }
+      LQ_TRY(launch_reduce_partials(ctx->stream, part, M, grid, kp, sums));
+      LQ_HIP_CHECK(hipMemcpyAsync(h.data(), sums, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+      // evolve phi_L and phi_R by dt: Phi <- e0 sum_k coef_k T_k(H~) Phi
+      LQ_TRY(launch_copy(ctx->stream, c0, pl, vec));
+      LQ_TRY(launch_copy(ctx->stream, d0, pr, vec));
+      {
+        VecSet sets[2] = {VecSet{c0, nullptr, c1, gg}, VecSet{d0, nullptr, d1, gg}};
+        LQ_TRY(spmm(ctx, H, one, zero, sets, 2));
+      }
+      double2 *x0 = c0, *x1 = c1, *y0 = d0, *y1 = d1;
+      {
+        SpmvLoopGuard guard{ctx};
+        LQ_TRY(spmv_loop_begin(ctx));
+        for (size_t k = 0; k < coef.size(); ++k) {
+          const std::complex<double> ck = coef[k] * e0;
+          LQ_TRY(launch_caxpy(ctx->stream, d2(ck.real(), ck.imag()), x0, pl, vec, k == 0));
+          LQ_TRY(launch_caxpy(ctx->stream, d2(ck.real(), ck.imag()), y0, pr, vec, k == 0));
+          if (k + 1 == coef.size()) break;
+          VecSet sets[2] = {VecSet{x1, x0, x0, gg}, VecSet{y1, y0, y0, gg}};
+          LQ_TRY(spmm(ctx, H, two, mone, sets, 2));
+          std::swap(x0, x1);
+          std::swap(y0, y1);
+        }
+        LQ_TRY(spmv_loop_end(ctx));
+      }
+      LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      for (int m = 0; m < M; ++m) {
+        double scal = 2.0 / R;
+        if (m == 0) scal *= 0.5;
+        for (int v = 0; v < gg; ++v)
+          acc[(size_t)m * nT + t] += scal * std::complex<double>(h[(size_t)m * kp + v].x, h[(size_t)m * kp + v].y);
+      }
+    }
+  }
+  for (size_t i = 0; i < acc.size(); ++i) {
+    mu[2 * i] += acc[i].real();
+    mu[2 * i + 1] += acc[i].imag();
+  }
+  return collect_timing(ctx);
+}
+
+int lq_jackson_td(lq_ctx ctx, double *mu, int M, int nT, double W, double broad_meV, int *M_out) {
+  if (!ctx || !mu || M <= 0 || nT <= 0 || !(broad_meV > 0) || !(W > 0) || !M_out) {
+    set_error("lq_jackson_td: invalid arguments");
+    return LQ_ERR_ARG;
+  }
+  DeviceGuard dg(ctx->device);
+  const double eta = 2.0 * broad_meV / 1000 / W;
+  int maxMom = (int)ceil(kPi / eta);
+  if (maxMom > M) maxMom = M;
+  double2 *d;
+  LQ_TRY(ctx_buf(ctx, "jk_td", (size_t)maxMom * nT * sizeof(double2), (void **)&d));
+  const size_t bytes = (size_t)maxMom * nT * sizeof(double2);  // rows m < maxMom are the first maxMom*nT
+  LQ_HIP_CHECK(hipMemcpyAsync(d, mu, bytes, hipMemcpyHostToDevice, ctx->stream));
+  LQ_TRY(launch_jackson_td(ctx->stream, d, maxMom, nT));
+  LQ_HIP_CHECK(hipMemcpyAsync(mu, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  *M_out = maxMom;
+  return LQ_OK;
+}
+
+int lq_timecorr_post(lq_ctx ctx, const double *mu, int M, int nT, int n, double W, double Ec, double dt, double EF,
+                     double *t, double *out) {
+  if (!ctx || !mu || M <= 0 || nT <= 0 || !(W > 0) || !t || !out) {
+    set_error("lq_timecorr_post: invalid arguments");
+    return LQ_ERR_ARG;
+  }
+  DeviceGuard dg(ctx->device);
+  double2 *d;
+  double *o;
+  LQ_TRY(ctx_buf(ctx, "tc_mu", (size_t)M * nT * sizeof(double2), (void **)&d));
+  LQ_TRY(ctx_buf(ctx, "tc_out", (size_t)2 * nT * sizeof(double), (void **)&o));
+  LQ_HIP_CHECK(hipMemcpyAsync(d, mu, (size_t)M * nT * sizeof(double2), hipMemcpyHostToDevice, ctx->stream));
+  LQ_TRY(launch_timecorr_post(ctx->stream, d, M, nT, n, W, Ec, dt, EF, o, o + nT));
+  LQ_HIP_CHECK(hipMemcpyAsync(t, o, nT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LQ_HIP_CHECK(hipMemcpyAsync(out, o + nT, nT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return LQ_OK;
+}
+
 int lq_kubo_post(lq_ctx ctx, int formula, const double *final_data, int nump, int n, double W, double Ec, double *E,
                  double *sigma, double *integrand) {
   if (!ctx || !final_data || nump <= 1 || n <= 0 || !E || !sigma ||

@@ -183,6 +183,11 @@ int launch_greenwood_fftgrid(hipStream_t st, const double2 *mu, int M0, int M1, 
                              int num_div, double *scratch, double *E, double *out);
 int launch_bastin_fftgrid(hipStream_t st, const double2 *mu, int M0, int M1, int n, double W, double Ec,
                           int num_div, double *scratch, double *E, double *out);
+// Time evolution and time-correlation post-processing (lq_post.hip).
+int launch_caxpy(hipStream_t st, double2 c, const double2 *x, double2 *y, size_t count, int overwrite);
+int launch_jackson_td(hipStream_t st, double2 *mu, int maxMom, int nT);
+int launch_timecorr_post(hipStream_t st, const double2 *mu, int M, int nT, int n, double W, double Ec, double dt,
+                         double EF, double *t, double *out);
 // Linear-grid tools (kind = LQ_LIN_*): kern scratch of num_div doubles; E, out
 // receive num_div (spectral) or num_div - 1 rows.
 int launch_linear_grid(hipStream_t st, int kind, const double2 *mu, int M0, int M1, int n, double W, double Ec,

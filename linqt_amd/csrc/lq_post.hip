@@ -383,6 +383,44 @@ __global__ void k_linear_finish(int kind, int num_div, double a, double Ec, cons
   }
 }
 
+// y (=|+=) c x over count complex elements (Bessel-Chebyshev time evolution).
+__global__ void k_caxpy(double2 c, const double2 *__restrict__ x, double2 *__restrict__ y, size_t count,
+                        int overwrite) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const double2 v = x[i];
+  double2 r = make_double2(c.x * v.x - c.y * v.y, c.x * v.y + c.y * v.x);
+  if (!overwrite) {
+    const double2 o = y[i];
+    r.x += o.x;
+    r.y += o.y;
+  }
+  y[i] = r;
+}
+
+// MomentsTD::ApplyJacksonKernel (src/chebyshev_momentsTD.cpp:135-154): row m of
+// the M x nT table (rows kept: maxMom) times g_m.
+__global__ void k_jackson_td(double2 *__restrict__ mu, int maxMom, int nT) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)maxMom * nT) return;
+  const int m = (int)(t / nT);
+  const double phi_J = kPi / (double)(maxMom + 1.0);
+  const double g = ((maxMom - m + 1) * cos(phi_J * m) + sin(phi_J * m) * cos(phi_J) / sin(phi_J)) * phi_J / kPi;
+  mu[t].x *= g;
+  mu[t].y *= g;
+}
+
+// timeCorrelationsFromChebmom.cpp:60-70: one thread per time step.
+__global__ void k_timecorr_post(const double2 *__restrict__ mu, int M, int nT, int n, double a, double x, double dt,
+                                double *__restrict__ tout, double *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nT) return;
+  double o = 0.0;
+  for (int m = 0; m < M; ++m) o += delta_chebF(x, m) * mu[(size_t)m * nT + t].x;
+  tout[t] = t * dt;
+  out[t] = o * (n / a);
+}
+
 __global__ void k_greenwood_energies(int num_div, double *__restrict__ e) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < num_div) e[i] = cos(kPi * (2 * i + 0.5) / num_div);
@@ -443,6 +481,29 @@ int launch_linear_grid(hipStream_t st, int kind, const double2 *mu, int M0, int 
 #undef LQ_LIN
   }
   hipLaunchKernelGGL(k_linear_finish, dim3(1), dim3(64), 0, st, kind, num_div, a, Ec, kern, E, out);
+  LQ_HIP_CHECK(hipGetLastError());
+  return LQ_OK;
+}
+
+int launch_caxpy(hipStream_t st, double2 c, const double2 *x, double2 *y, size_t count, int overwrite) {
+  hipLaunchKernelGGL(k_caxpy, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, c, x, y, count, overwrite);
+  LQ_HIP_CHECK(hipGetLastError());
+  return LQ_OK;
+}
+
+int launch_jackson_td(hipStream_t st, double2 *mu, int maxMom, int nT) {
+  const long long total = (long long)maxMom * nT;
+  if (total <= 0) return LQ_OK;
+  hipLaunchKernelGGL(k_jackson_td, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, mu, maxMom, nT);
+  LQ_HIP_CHECK(hipGetLastError());
+  return LQ_OK;
+}
+
+int launch_timecorr_post(hipStream_t st, const double2 *mu, int M, int nT, int n, double W, double Ec, double dt,
+                         double EF, double *t, double *out) {
+  const double a = W / 2.0;
+  hipLaunchKernelGGL(k_timecorr_post, dim3((nT + 127) / 128), dim3(128), 0, st, mu, M, nT, n, a, (EF - Ec) / a, dt, t,
+                     out);
   LQ_HIP_CHECK(hipGetLastError());
   return LQ_OK;
 }

@@ -103,6 +103,27 @@ def read_chebmom1d(path: str):
     return v[:, 0] + 1j * v[:, 1], n, W, Ec
 
 
+def read_chebmomtd(path: str):
+    """``MomentsTD(std::string)`` (src/chebyshev_momentsTD.cpp:48-79); returns
+    (mu (M, nT), N, W, Ec, dt)."""
+    if ".chebmomTD" not in path:
+        raise ValueError("The first argument does not seem to be a valid .chebmomTD file")
+    t = _tokens(path)
+    n, W, Ec, _, dt = int(t[0]), float(t[1]), float(t[2]), int(t[3]), float(t[4])
+    M, nT = int(t[5]), int(t[6])
+    v = t[7 : 7 + 2 * M * nT].reshape(M * nT, 2)
+    return (v[:, 0] + 1j * v[:, 1]).reshape(M, nT), n, W, Ec, dt
+
+
+def write_chebmomtd(path: str, mu: np.ndarray, n: int, W: float, Ec: float, dt: float) -> None:
+    """``MomentsTD::saveIn`` (src/chebyshev_momentsTD.cpp:81-93)."""
+    M, nT = mu.shape
+    with open(path, "w") as f:
+        f.write(f"{n} {_fmt(W)} {_fmt(Ec)} {nT} {_fmt(dt)} \n{M} {nT} \n")
+        for z in mu.reshape(-1):
+            f.write(f"{_fmt(z.real)} {_fmt(z.imag)}\n")
+
+
 def write_chebmom1d(path: str, mu: np.ndarray, n: int, W: float, Ec: float) -> None:
     """``Moments1D::saveIn`` (``src/chebyshev_moments1D.cpp:91-103``)."""
     with open(path, "w") as f:

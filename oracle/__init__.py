@@ -52,6 +52,11 @@ def lib():
         L.lqo_kubo_post.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp]
         L.lqo_jackson_1d.argtypes = [_dp, C.c_int, C.c_double, C.c_double]
         L.lqo_jackson_2d.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _ip, _ip]
+        L.lqo_time_correlations.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _dp, C.c_int, C.c_int,
+                                            C.c_int, C.c_double, C.c_double, C.c_double, _dp]
+        L.lqo_jackson_td.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_double, _ip]
+        L.lqo_timecorr_post.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, _dp, _dp]
         L.lqo_linear_kernel.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp]
         L.lqo_linear_grid.argtypes = [C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                       _dp, _dp]
@@ -210,6 +215,31 @@ def bastin_fftgrid(mu: np.ndarray, n: int, W: float, Ec: float, num_div: int):
     _check(lib().lqo_bastin_fftgrid(_d(mu), mu.shape[0], mu.shape[1], n, W, Ec, num_div, _d(E), _d(out)),
            "bastin_fftgrid")
     return E, out
+
+
+def time_correlations(H: Op, OPL, OPR, phi: np.ndarray, M: int, nT: int, dt: float, W: float, Ec: float,
+                      mu=None) -> np.ndarray:
+    """TimeDependentCorrelations: (M, nT) table, accumulated into mu when given."""
+    phi = np.ascontiguousarray(np.atleast_2d(phi), dtype=np.complex128)
+    out = np.zeros((M, nT), dtype=np.complex128) if mu is None else np.array(mu, dtype=np.complex128)
+    _check(lib().lqo_time_correlations(H.h, None if OPL is None else OPL.h, None if OPR is None else OPR.h, H.n,
+                                       _d(phi), phi.shape[0], M, nT, dt, W, Ec, _d(out)), "time_correlations")
+    return out
+
+
+def jackson_td(mu: np.ndarray, W: float, broad: float) -> np.ndarray:
+    mu = np.array(mu, dtype=np.complex128, order="C")
+    mo = C.c_int(0)
+    _check(lib().lqo_jackson_td(_d(mu), mu.shape[0], mu.shape[1], W, broad, C.byref(mo)), "jackson_td")
+    return mu[: mo.value].copy()
+
+
+def timecorr_post(mu: np.ndarray, n: int, W: float, Ec: float, dt: float, EF: float):
+    mu = np.ascontiguousarray(mu, dtype=np.complex128)
+    t, o = np.zeros(mu.shape[1]), np.zeros(mu.shape[1])
+    _check(lib().lqo_timecorr_post(_d(mu), mu.shape[0], mu.shape[1], n, W, Ec, dt, EF, _d(t), _d(o)),
+           "timecorr_post")
+    return t, o
 
 
 LINEAR_KINDS = {"spectral": 0, "greenwood": 1, "bastin": 2, "bastinI": 3, "bastinII": 4, "sea": 5, "surf": 6}

@@ -247,6 +247,109 @@ int lqo_spectral_moments(const lqo_op *H, const lqo_op *OP, int n, const double 
   return 0;
 }
 
+/* ---- time-dependent correlations ----------------------------------------- */
+static double delta_chebF(double x, double m);
+#define LQO_HBAR 0.6582119624 /* eV fs (include/chebyshev_moments.hpp:290) */
+
+/* chebyshev::besselJ (src/special_functions.cpp:3-17): libm's j0, j1, jn. */
+static double bessel_j(int n, double x) { return n == 0 ? j0(x) : n == 1 ? j1(x) : jn(n, x); }
+
+/* MomentsTD::Evolve (src/chebyshev_momentsTD.cpp:4-45): Phi <- exp(-i H dt/hbar) Phi
+ * as sum_n c_n T_n(H~) Phi, c_0 = J_0(x), c_n = 2 (-i)^n J_n(x), x = dt a/hbar, terms
+ * added while |J_n| > 1e-10, then the phase exp(-i Ec dt/hbar). */
+static void evolve(cheb_state *s, cplx *Phi, double a, double Ec, double dt) {
+  const int n = s->n;
+  const double x = dt * (a / LQO_HBAR), tol = 1e-10;
+  double momcut = 2.0 * tol;
+  memcpy(s->c0, Phi, sizeof(cplx) * (size_t)n);
+  op_mv_c(s->H, 1.0, s->c0, 0.0, s->c1);
+  int k = 0;
+  cplx nIp = 1.0;
+  double Jn = 0.5 * bessel_j(k, x);
+  for (int i = 0; i < n; ++i) Phi[i] = 0.0;
+  while (momcut > tol) {
+    const cplx c = nIp * 2.0 * Jn;
+    for (int i = 0; i < n; ++i) Phi[i] += c * s->c0[i];
+    iterate(s);
+    nIp *= -I;
+    ++k;
+    Jn = bessel_j(k, x);
+    momcut = fabs(Jn);
+  }
+  const cplx e0 = cexp(-I * (Ec / LQO_HBAR) * dt);
+  for (int i = 0; i < n; ++i) Phi[i] *= e0;
+}
+
+/* chebyshev::TimeDependentCorrelations (src/chebyshev_solver.cpp:231-275). */
+int lqo_time_correlations(const lqo_op *H, const lqo_op *OPL, const lqo_op *OPR, int n, const double *phi, int R,
+                          int M, int nT, double dt, double W, double Ec, double *mu_) {
+  if (!H || H->n != n || (OPL && OPL->n != n) || (OPR && OPR->n != n) || !phi || R <= 0 || M <= 0 || nT <= 0 ||
+      !mu_)
+    return -1;
+  cplx *mu = (cplx *)mu_;
+  const double a = W / 2.0;
+  cheb_state s = {H, n, (cplx *)malloc(sizeof(cplx) * (size_t)n), (cplx *)malloc(sizeof(cplx) * (size_t)n)};
+  cplx *PhiR = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+  cplx *PhiL = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+  for (int r = 0; r < R; ++r) {
+    memcpy(PhiR, (const cplx *)phi + (size_t)r * n, sizeof(cplx) * (size_t)n);
+    if (OPL)
+      op_mv_c(OPL, 1.0, PhiR, 0.0, PhiL);
+    else
+      memcpy(PhiL, PhiR, sizeof(cplx) * (size_t)n);
+    for (int t = 0; t < nT; ++t) {
+      set_init_op(&s, OPR, PhiR);
+      for (int m = 0; m < M; ++m) {
+        double scal = 2.0 / R;
+        if (m == 0) scal *= 0.5;
+        mu[(size_t)m * nT + t] += scal * vdot(n, PhiL, s.c0);
+        iterate(&s);
+      }
+      evolve(&s, PhiL, a, Ec, dt);
+      evolve(&s, PhiR, a, Ec, dt);
+    }
+  }
+  free(s.c0);
+  free(s.c1);
+  free(PhiR);
+  free(PhiL);
+  return 0;
+}
+
+/* MomentsTD::ApplyJacksonKernel (src/chebyshev_momentsTD.cpp:135-154): truncate
+ * to ceil(pi/eta) moments (eta = 2 b/1000/W) and damp every time column. */
+int lqo_jackson_td(double *mu_, int M, int nT, double W, double broad, int *M_out) {
+  if (!mu_ || M <= 0 || nT <= 0 || broad <= 0 || !M_out) return -1;
+  cplx *mu = (cplx *)mu_;
+  const double eta = 2.0 * broad / 1000 / W;
+  int maxMom = (int)ceil(M_PI / eta);
+  if (maxMom > M) maxMom = M;
+  const double phi_J = M_PI / (double)(maxMom + 1.0);
+  for (int m = 0; m < maxMom; ++m) {
+    const double g = ((maxMom - m + 1) * cos(phi_J * m) + sin(phi_J * m) * cos(phi_J) / sin(phi_J)) * phi_J / M_PI;
+    for (int t = 0; t < nT; ++t) mu[(size_t)m * nT + t] *= g;
+  }
+  *M_out = maxMom;
+  return 0;
+}
+
+/* timeCorrelationsFromChebmom.cpp:41-70: rows (n dt, N/a sum_m delta_m(x_F) Re mu(m, n))
+ * with x_F = (E_F - Ec)/a; mu holds M (truncated) x nT moments, row stride nT. */
+int lqo_timecorr_post(const double *mu_, int M, int nT, int n, double W, double Ec, double dt, double EF,
+                      double *t_out, double *out) {
+  if (!mu_ || M <= 0 || nT <= 0 || !t_out || !out) return -1;
+  const cplx *mu = (const cplx *)mu_;
+  const double a = W / 2.0, x = (EF - Ec) / a;
+  for (int t = 0; t < nT; ++t) {
+    double o = 0.0;
+    for (int m = 0; m < M; ++m) o += delta_chebF(x, m) * creal(mu[(size_t)m * nT + t]);
+    o *= n / a;
+    t_out[t] = t * dt;
+    out[t] = o;
+  }
+  return 0;
+}
+
 /* ---- stored Chebyshev vectors: Vectors_sliced / Vectors ---------------- */
 typedef struct {
   cheb_state st;

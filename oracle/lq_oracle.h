@@ -98,6 +98,19 @@ int lqo_greenwood_fftgrid(const double *mu, int M0, int M1, int n, double W,
 int lqo_bastin_fftgrid(const double *mu, int M0, int M1, int n, double W,
                        double Ec, int num_div, double *E_out, double *out);
 
+/* chebyshev::TimeDependentCorrelations (src/chebyshev_solver.cpp:231-275):
+ * mu (M x nT complex, row m stride nT) += (2/R)(1/2)^{delta_m0}
+ * <OPL phi(t_n)|T_m(H~) OPR|phi(t_n)> with phi(t) evolved by MomentsTD::Evolve
+ * (Bessel-Chebyshev expansion of exp(-iHt/hbar)), t_n = n dt (fs); H already
+ * rescaled with W, Ec (the frequencies of the expansion).  OPL/OPR NULL = 1. */
+int lqo_time_correlations(const lqo_op *H, const lqo_op *OPL, const lqo_op *OPR, int n, const double *phi,
+                          int R, int M, int nT, double dt, double W, double Ec, double *mu);
+/* MomentsTD::ApplyJacksonKernel; *M_out = truncated moment count (rows kept). */
+int lqo_jackson_td(double *mu, int M, int nT, double W, double broad_meV, int *M_out);
+/* timeCorrelationsFromChebmom.cpp: nT rows (t, value) at Fermi energy EF. */
+int lqo_timecorr_post(const double *mu, int M, int nT, int n, double W, double Ec, double dt, double EF,
+                      double *t_out, double *out);
+
 /* Linear-grid post-processors (src/...FromChebmom.cpp): kind 0 spectral
  * function (1D mu of M0 moments), 1 Greenwood, 2 Bastin, 3 BastinI,
  * 4 BastinII, 5 BastinSea, 6 BastinSurf (2D mu, M0 x M1).

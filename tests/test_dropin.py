@@ -36,7 +36,8 @@ def test_drivers_print_usage_without_gpu():
                 "spectralFunctionFromChebmom_FFTgrid", "kuboGreenwoodFromChebmom_FFTgrid",
                 "kuboBastinFromChebmom_FFTgrid", "spectralFunctionFromChebmom", "kuboGreenwoodFromChebmom",
                 "kuboBastinFromChebmom", "kuboBastinIFromChebmom", "kuboBastinIIFromChebmom",
-                "kuboBastinSeaFromChebmom", "kuboBastinSurfFromChebmom"):
+                "kuboBastinSeaFromChebmom", "kuboBastinSurfFromChebmom", "inline_compute-kpm-CorrOp",
+                "timeCorrelationsFromChebmom"):
         r = subprocess.run([os.path.join(BIN, exe)], capture_output=True, text=True, timeout=60)
         assert r.returncode == 0 and "should be called" in r.stdout, exe
 
@@ -164,3 +165,28 @@ def test_linear_grid_drivers_match_oracle(chain_dir, oracle):
     assert len(E) == len(Eo) and np.array_equal(np.isfinite(v), np.isfinite(vo))
     f = np.isfinite(vo)
     assert np.abs(v[f] - vo[f]).max() <= 2e-5 * np.abs(vo[f]).max()
+
+
+@pytest.mark.gpu
+def test_time_correlation_drivers_match_oracle(chain_dir, oracle):
+    """inline_compute-kpm-CorrOp (default state: site N/2) and timeCorrelationsFromChebmom on
+    the reference's linear chain."""
+    (chain_dir / "BOUNDS").write_text("-5 5\n")
+    M, nT, tmax = 40, 5, 2.0
+    _run("inline_compute-kpm-CorrOp", ["linear_chain", "VX", "VX", M, nT, tmax], chain_dir)
+    out = chain_dir / f"TimeCorrVX-VXlinear_chainKPM_M{M}_statedefault.chebmomTD"
+    mu, n, W, Ec, dt = io.read_chebmomtd(str(out))
+    H = io.read_csr(os.path.join(LC, "operators", "linear_chain.HAM.CSR"))
+    V = io.read_csr(os.path.join(LC, "operators", "linear_chain.VX.CSR"))
+    Ho = oracle.Op.from_csr(H).rescale(1 / 5.0, 0.0)
+    Vo = oracle.Op.from_csr(V)
+    phi = io.default_state(H.n)
+    ref = oracle.time_correlations(Ho, Vo, Vo, phi, M, nT, tmax / (nT - 1), 10.0, 0.0)
+    assert (n, W, Ec) == (H.n, 10.0, 0.0) and abs(dt - tmax / (nT - 1)) < 1e-14
+    assert rel_inf(mu, ref) < 1e-12
+    _run("timeCorrelationsFromChebmom", [out.name, 100, 0.3], chain_dir)
+    rows = sorted(chain_dir.glob("meanTimeCorr*EF0.300000JACKSON.dat"))
+    assert len(rows) == 1
+    t, v = _two_col(rows[0])
+    to, vo = oracle.timecorr_post(oracle.jackson_td(mu, W, 100.0), n, W, Ec, dt, 0.3)
+    assert np.allclose(t, to, rtol=1e-5) and rel_inf(v, vo) < 1e-5

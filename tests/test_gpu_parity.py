@@ -305,3 +305,29 @@ def test_linear_grid_tools_vs_oracle(ctx, oracle, kind):
         muJ = muJ[:, 0].copy()
     nd = 30 * max(np.atleast_2d(muJ).shape)
     _same_rows(lq.linear_grid(ctx, kind, muJ, gn, gW, gEc), oracle.linear_grid(kind, muJ, gn, gW, gEc, nd), 1e-9)
+
+
+def test_time_correlations_vs_oracle(ctx, oracle):
+    """TimeDependentCorrelations with V_x, V_y and with identities, several states,
+    a band centre != 0 (the propagator's phase), then the TD Jackson kernel and
+    the time-correlation post-processor."""
+    say("time correlations")
+    H, Vx, Vy = lattices.haldane_honeycomb(6, seed=21)
+    W, Ec = 9.4, 0.15
+    a = W / 2
+    G = lq.SparseMatrix.from_csr(ctx, H).Rescale(1 / a, -Ec / a)
+    O = oracle.Op.from_csr(H).rescale(1 / a, -Ec / a)
+    GV = [lq.SparseMatrix.from_csr(ctx, Vx), lq.SparseMatrix.from_csr(ctx, Vy)]
+    OV = [oracle.Op.from_csr(Vx), oracle.Op.from_csr(Vy)]
+    phi = lattices.random_phase_states(H.n, 3, seed=22)
+    M, nT, dt = 30, 6, 1.3
+    for (gl, gr, ol, orr) in ((GV[0], GV[1], OV[0], OV[1]), (None, None, None, None)):
+        got = lq.time_correlations(G, gl, gr, phi, M, nT, dt, W, Ec)
+        ref = oracle.time_correlations(O, ol, orr, phi, M, nT, dt, W, Ec)
+        assert rel_inf(got, ref) < 1e-10
+    tdj = lq.jackson_td(ctx, got, W, 400.0)
+    tdo = oracle.jackson_td(ref, W, 400.0)
+    assert tdj.shape == tdo.shape and rel_inf(tdj, tdo) < 1e-10
+    t, v = lq.timecorr_post(ctx, tdj, H.n, W, Ec, dt, 0.7)
+    to, vo = oracle.timecorr_post(tdo, H.n, W, Ec, dt, 0.7)
+    assert np.allclose(t, to) and rel_inf(v, vo) < 1e-9

@@ -424,3 +424,57 @@ def test_linear_grid_rows_and_end_points(oracle):
             assert np.isnan(o[0]) and np.all(np.isfinite(o[1:-1]))
     E, o = oracle.linear_grid(0, mu[:, 0], n, W, Ec, nd)
     assert len(E) == nd and np.isnan(o[0]) and np.isnan(o[-1]) and np.all(np.isfinite(o[1:-1]))
+
+
+# ---- time-dependent correlations -------------------------------------------------------
+
+HBAR = 0.6582119624
+
+
+def test_time_correlations_match_eigendecomposition(oracle):
+    """mu(m, n) = sum_r (2/R)(1/2)^{d_m0} (OPL phi)^H e^{iHt_n} T_m(H~) OPR e^{-iHt_n} phi with
+    the exact propagator from eigh (the Bessel-Chebyshev evolution truncates at |J_n| <= 1e-10);
+    a band centre Ec != 0 checks the phase factor, OPL = OPR = None the identities."""
+    n = 40
+    rowptr, col, val = random_upper_csr(n, 0.2, seed=8)
+    Hd = dense_herm(n, rowptr, col, val)
+    e = np.linalg.eigvalsh(Hd)
+    Ec, a = 0.37, (e.max() - e.min()) / 2 * 1.05 + 0.1
+    W = 2 * a
+    H = oracle.Op(n, rowptr, col, val).rescale(1 / a, -Ec / a)
+    r2, c2, v2 = random_upper_csr(n, 0.15, seed=9)
+    r3, c3, v3 = random_upper_csr(n, 0.15, seed=10)
+    OL, OR = oracle.Op(n, r2, c2, v2), oracle.Op(n, r3, c3, v3)
+    Ld, Rd = dense_herm(n, r2, c2, v2), dense_herm(n, r3, c3, v3)
+    rng = np.random.default_rng(3)
+    phi = rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n))
+    M, nT, dt = 12, 5, 1.7
+    E, U = np.linalg.eigh(Hd)
+    for OPL, OPR, Ldn, Rdn in ((OL, OR, Ld, Rd), (None, None, np.eye(n), np.eye(n))):
+        got = oracle.time_correlations(H, OPL, OPR, phi, M, nT, dt, W, Ec)
+        ref = np.zeros((M, nT), dtype=np.complex128)
+        for t in range(nT):
+            Ut = U @ np.diag(np.exp(-1j * E * t * dt / HBAR)) @ U.conj().T
+            for m in range(M):
+                Tm = U @ np.diag(np.cos(m * np.arccos((E - Ec) / a))) @ U.conj().T
+                s = 2.0 / 2 * (0.5 if m == 0 else 1.0)
+                for r in range(2):
+                    pl = Ut @ (Ldn @ phi[r])
+                    pr = Ut @ phi[r]
+                    ref[m, t] += s * np.vdot(pl, Tm @ (Rdn @ pr))
+        assert rel_inf(got, ref) < 1e-8
+
+
+def test_jackson_td_and_time_post(oracle):
+    rng = np.random.default_rng(6)
+    mu = rng.normal(size=(50, 7)) + 1j * rng.normal(size=(50, 7))
+    W, Ec, n, dt = 6.0, 0.2, 300, 0.5
+    td = oracle.jackson_td(mu, W, 120.0)
+    for t in range(7):
+        assert rel_inf(td[:, t], oracle.jackson_1d(mu[:, t].copy(), W, 120.0)) < 1e-15
+    EF = 0.9
+    tt, o = oracle.timecorr_post(td, n, W, Ec, dt, EF)
+    assert np.allclose(tt, dt * np.arange(7))
+    x = np.array([(EF - Ec) / (W / 2)])
+    for t in range(7):
+        assert abs(o[t] - oracle.linear_kernel(0, td[:, t].copy(), n, W, x)[0]) < 1e-12 * abs(o[t]) + 1e-14

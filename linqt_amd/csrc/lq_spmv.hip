@@ -314,7 +314,10 @@ __global__ __launch_bounds__(256) void k_spmm_batch(const BatchJobDev *__restric
   const int groups = (J.n + RPB - 1) / RPB;
   double2 d[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
   if (J.ndict > 0) {
-    const DictMat mat{J.dsptr, J.wcol, J.dict, J.diag};
+    __shared__ double2 sdict[256];
+    stage(sdict, J.dict, J.ndict);  // J is the same for the whole block
+    __syncthreads();
+    const DictMat mat{J.dsptr, J.wcol, sdict, J.diag};
     spmm_body<KP, kDotPhi, kUnroll>(J.n, mat, alpha, beta, sa, groups, J.phi, lb, J.nblk, d);
   } else {
     const PlainMat mat{J.sptr, J.col, J.val};

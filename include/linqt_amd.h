@@ -243,6 +243,48 @@ int lq_batch_result(lq_batch b, int job, double *mu);
 /* Gather unroll the tuner chose for jobs with R states (0: still tuning). */
 int lq_batch_unroll(lq_batch b, int R, int *unroll);
 
+/* ---- file ingestion (host side, no GPU needed) ------------------------------ */
+/* File formats of the writers.  Every reader accepts the text layout and the
+ * binary one (told apart by the binary magic). */
+enum lq_file_format {
+  LQ_FMT_TEXT = 0,     /* the reference's text layout, 17 significant digits (exact round trip) */
+  LQ_FMT_TEXT_REF = 1, /* the reference's text layout at its writers' precision: 6 digits for
+                          CSR (ofstream default), 15 for moment files (precision(digits10)) */
+  LQ_FMT_BINARY = 2    /* "LQCSR001" / "LQMOM001" binary, little endian */
+};
+
+/* Header of a moment file: kind 1 = .chebmom1D (M1 = 1), 2 = .chebmom2D,
+ * 3 = .chebmomTD (M0 moments x M1 time steps, dt set). */
+typedef struct lq_mom_header {
+  int kind;
+  int M0, M1;
+  long long n;
+  double W, Ec, dt;
+} lq_mom_header;
+
+/* Replaces Sparse::OPERATOR_FromCSRFile (src/sparse_matrix.cpp:3-50): the
+ * CSR file (text or binary) as zero-based host arrays.  Call with rowptr ==
+ * NULL for n and nnz only, then with arrays of n+1, nnz and 2*nnz entries.
+ * Text is parsed by several threads (LQ_IO_THREADS caps them); every value is
+ * bitwise what `ifstream >> double` reads. */
+int lq_csr_file_read(const char *path, int *n, long long *nnz, int *rowptr, int *col, double *val);
+/* Writes a CSR file; LQ_FMT_TEXT_REF reproduces
+ * hopping_list::save_hopping_list_as_csr byte for byte
+ * (utilities/wannier2sparse/src/hopping_list.cpp:262-283). */
+int lq_csr_file_write(const char *path, int format, int n, long long nnz, const int *rowptr, const int *col,
+                      const double *val);
+/* Replaces SparseMatrixBuilder::BuildOPFromCSRFile
+ * (include/sparse_matrix.hpp:100-109): lq_csr_file_read + lq_csr_create. */
+int lq_csr_file_load(lq_ctx ctx, const char *path, int fill, lq_csr *out);
+/* Replaces the Moments1D / Moments2D / MomentsTD file constructors
+ * (src/chebyshev_moments1D.cpp:19-51, src/chebyshev_moments2D.cpp:16-51,
+ * src/chebyshev_momentsTD.cpp:48-79): header, then (mu != NULL) the
+ * M0*M1 complex moments, row major. */
+int lq_mom_file_read(const char *path, lq_mom_header *h, double *mu);
+/* Moments1D/2D/TD::saveIn (src/chebyshev_moments1D.cpp:91-103,
+ * src/chebyshev_moments2D.cpp:53-68, src/chebyshev_momentsTD.cpp:81-93). */
+int lq_mom_file_write(const char *path, int format, const lq_mom_header *h, const double *mu);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,10 +29,11 @@ import numpy as np
 from . import _lib
 from ._lib import (LQ_ALGO_DIRECT, LQ_ALGO_DOUBLING, LQ_FILL_GENERAL, LQ_FILL_HERMITIAN_UPPER,
                    LQ_KUBO_BASTIN, LQ_KUBO_GREENWOOD, LQ_PREC_FP64, LQ_PREC_INT8X4, LQ_PREC_INT8X5,
-                   LQ_PREC_INT8X6, LQ_PREC_INT8X7, LinqtError)
+                   LQ_PREC_INT8X6, LQ_PREC_INT8X7, LQ_FMT_TEXT, LQ_FMT_TEXT_REF, LQ_FMT_BINARY, LinqtError)
 
 __all__ = [
-    "Context", "SparseMatrix", "DeviceStates", "Batch", "linear_grid", "time_correlations", "jackson_td",
+    "Context", "SparseMatrix", "DeviceStates", "Batch", "read_csr_file", "write_csr_file", "read_mom_file",
+    "write_mom_file", "linear_grid", "time_correlations", "jackson_td",
     "timecorr_post", "spectral_moments", "correlation_expansion_moments", "kubo_fft", "kubo_post",
     "jackson_1d", "jackson_2d", "dos_fftgrid", "greenwood_fftgrid", "bastin_fftgrid", "LinqtError",
     "LQ_FILL_HERMITIAN_UPPER", "LQ_FILL_GENERAL", "LQ_KUBO_GREENWOOD", "LQ_KUBO_BASTIN", "LQ_ALGO_DIRECT",
@@ -139,10 +140,8 @@ class SparseMatrix:
 
     @classmethod
     def from_file(cls, ctx: Context, path: str, id: str = ""):
-        """``SparseMatrixBuilder::BuildOPFromCSRFile``."""
-        from .io import read_csr
-
-        return cls.from_csr(ctx, read_csr(path), id=id)
+        """``SparseMatrixBuilder::BuildOPFromCSRFile`` (text or binary CSR, parsed natively)."""
+        return cls.from_csr(ctx, read_csr_file(path), id=id)
 
     def isIdentity(self) -> bool:
         return self.id == "1"
@@ -422,3 +421,61 @@ def bastin_fftgrid(ctx: Context, mu: np.ndarray, n: int, W: float, Ec: float, nu
     _lib.call("lq_bastin_fftgrid", ctx.h, _dp(mu.view(np.float64)), mu.shape[0], mu.shape[1], n, W, Ec, num_div,
               _dp(E), _dp(s))
     return E, s
+
+
+# ---- file ingestion (host side; no GPU needed) --------------------------------
+def read_csr_file(path: str):
+    """``Sparse::OPERATOR_FromCSRFile`` (src/sparse_matrix.cpp:3-50) through the
+    native multi-threaded reader; text or binary CSR.  Returns ``io.CSR``."""
+    from .io import CSR
+
+    n, nnz = C.c_int(0), C.c_longlong(0)
+    bp = str(path).encode()
+    _lib.call("lq_csr_file_read", bp, C.byref(n), C.byref(nnz), None, None, None)
+    rowptr = np.zeros(n.value + 1, dtype=np.int32)
+    col = np.zeros(max(nnz.value, 1), dtype=np.int32)
+    val = np.zeros(max(nnz.value, 1), dtype=np.complex128)
+    _lib.call("lq_csr_file_read", bp, C.byref(n), C.byref(nnz), _ip(rowptr), _ip(col), _dp(val.view(np.float64)))
+    return CSR(n.value, rowptr, col[: nnz.value], val[: nnz.value])
+
+
+def write_csr_file(path: str, m, fmt: int = _lib.LQ_FMT_TEXT) -> None:
+    """Writes an ``io.CSR``; ``LQ_FMT_TEXT_REF`` is byte for byte what
+    ``hopping_list::save_hopping_list_as_csr`` writes (hopping_list.cpp:262-283)."""
+    rowptr = np.ascontiguousarray(m.rowptr, dtype=np.int32)
+    col = np.ascontiguousarray(m.col, dtype=np.int32)
+    val = np.ascontiguousarray(m.val, dtype=np.complex128)
+    if col.size == 0:
+        col, val = np.zeros(1, np.int32), np.zeros(1, np.complex128)
+    _lib.call("lq_csr_file_write", str(path).encode(), fmt, int(m.n), int(m.nnz), _ip(rowptr), _ip(col),
+              _dp(val.view(np.float64)))
+
+
+MOM_1D, MOM_2D, MOM_TD = 1, 2, 3
+
+
+def read_mom_file(path: str):
+    """Moments1D / Moments2D / MomentsTD file constructors, text or binary.
+    Returns (mu, header) with mu of shape (M,), (M0, M1) or (M, nT) and header
+    a dict with kind, n, W, Ec, dt."""
+    h = _lib.lq_mom_header()
+    bp = str(path).encode()
+    _lib.call("lq_mom_file_read", bp, C.byref(h), None)
+    mu = np.zeros(h.M0 * h.M1, dtype=np.complex128)
+    _lib.call("lq_mom_file_read", bp, C.byref(h), _dp(mu.view(np.float64)))
+    if h.kind != MOM_1D:
+        mu = mu.reshape(h.M0, h.M1)
+    return mu, {"kind": h.kind, "n": h.n, "W": h.W, "Ec": h.Ec, "dt": h.dt}
+
+
+def write_mom_file(path: str, mu: np.ndarray, n: int, W: float, Ec: float, kind: int | None = None,
+                   dt: float = 0.0, fmt: int = _lib.LQ_FMT_TEXT_REF) -> None:
+    """Moments1D/2D/TD::saveIn layouts (LQ_FMT_TEXT_REF: the reference's 15
+    digits), 17-digit text or binary."""
+    mu = np.ascontiguousarray(mu, dtype=np.complex128)
+    if kind is None:
+        kind = MOM_1D if mu.ndim == 1 else MOM_2D
+    M0, M1 = (mu.shape[0], 1) if mu.ndim == 1 else mu.shape
+    h = _lib.lq_mom_header(kind, M0, M1, n, W, Ec, dt)
+    _lib.call("lq_mom_file_write", str(path).encode(), fmt, C.byref(h), _dp(mu.reshape(-1).view(np.float64)))
+

@@ -27,6 +27,7 @@ LQ_ALGO_DOUBLING = 1
 LQ_PREC_FP64 = 0
 LQ_LIN_SPECTRAL, LQ_LIN_GREENWOOD, LQ_LIN_BASTIN, LQ_LIN_BASTIN_I, LQ_LIN_BASTIN_II, LQ_LIN_BASTIN_SEA, \
     LQ_LIN_BASTIN_SURF = range(7)
+LQ_FMT_TEXT, LQ_FMT_TEXT_REF, LQ_FMT_BINARY = range(3)
 LQ_PREC_INT8X4 = 4
 LQ_PREC_INT8X5 = 5
 LQ_PREC_INT8X6 = 6
@@ -46,6 +47,18 @@ class lq_timing(C.Structure):
         ("contract_flops", C.c_double),
         ("spmv_launches", C.c_longlong),
         ("contract_launches", C.c_longlong),
+    ]
+
+
+class lq_mom_header(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("M0", C.c_int),
+        ("M1", C.c_int),
+        ("n", C.c_longlong),
+        ("W", C.c_double),
+        ("Ec", C.c_double),
+        ("dt", C.c_double),
     ]
 
 
@@ -89,6 +102,11 @@ _SIGS = {
     "lq_batch_poll": (C.c_int, [_vp, C.c_int, _ip]),
     "lq_batch_result": (C.c_int, [_vp, C.c_int, _dp]),
     "lq_batch_unroll": (C.c_int, [_vp, C.c_int, _ip]),
+    "lq_csr_file_read": (C.c_int, [C.c_char_p, _ip, C.POINTER(C.c_longlong), _ip, _ip, _dp]),
+    "lq_csr_file_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_longlong, _ip, _ip, _dp]),
+    "lq_csr_file_load": (C.c_int, [_vp, C.c_char_p, C.c_int, C.POINTER(_vp)]),
+    "lq_mom_file_read": (C.c_int, [C.c_char_p, C.c_void_p, _dp]),
+    "lq_mom_file_write": (C.c_int, [C.c_char_p, C.c_int, C.c_void_p, _dp]),
     "lq_spectral_moments": (C.c_int, [_vp, _vp, _dp, C.c_int, C.c_int, C.c_int, _dp]),
     "lq_correlation_moments": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(_vp), _dp, C.c_int, C.c_int, C.c_int,
                                          C.c_int, C.c_int, _dp]),

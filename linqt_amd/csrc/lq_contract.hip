@@ -247,6 +247,140 @@ __global__ __launch_bounds__(512) void k_contract_f64_lds(const double2 *__restr
       }
 }
 
+// Three-multiplication variant (the default for tables of at least 128 x 128):
+// with P1 = sum Ar Br, P2 = sum Ai Bi and P3 = sum (Ar - Ai)(Br + Bi),
+//   Re C = P1 + P2,   Im C = P3 - P1 + P2,
+// three real MFMAs per complex product instead of four.  512 threads (two
+// waves per SIMD), block tile 128 (A rows) x 64 (B rows), waves 4 (m) x 2 (n)
+// of 32 x 32 (2 x 2 MFMA tiles, 3 x 4 accumulators = 96 registers), K in
+// stages of 16 complex elements double-buffered in LDS (rows padded to 17).
+// The sums Ar - Ai and Br + Bi are formed when a fragment is read.  Rounding:
+// Im carries the error of P3 (operands rounded once more) on top of the
+// four-multiplication form, a few units in the last place of sum |A||B|.
+constexpr int kBM3 = 128, kBN3 = 64;
+
+__device__ __forceinline__ void mma3_stage(const double2 *__restrict__ sA, const double2 *__restrict__ sB, int arow0,
+                                           int brow0, int ks, v4d (&p1)[2][2], v4d (&p2)[2][2], v4d (&p3)[2][2]) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    double ar[2], ai[2], ad[2], br[2], bi[2], bs[2];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti) {
+      const double2 x = sA[(arow0 + ti * 16) * kLdsRow + ks * 4 + s];
+      ar[ti] = x.x;
+      ai[ti] = x.y;
+      ad[ti] = x.x - x.y;
+    }
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) {
+      const double2 y = sB[(brow0 + tj * 16) * kLdsRow + ks * 4 + s];
+      br[tj] = y.x;
+      bi[tj] = y.y;
+      bs[tj] = y.x + y.y;
+    }
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj) {
+        p1[ti][tj] = mfma(ar[ti], br[tj], p1[ti][tj]);
+        p2[ti][tj] = mfma(ai[ti], bi[tj], p2[ti][tj]);
+        p3[ti][tj] = mfma(ad[ti], bs[tj], p3[ti][tj]);
+      }
+  }
+}
+
+__global__ __launch_bounds__(512) void k_contract_f64_3m(const double2 *__restrict__ A, long long lda, int M0,
+                                                         const double2 *__restrict__ B, long long ldb, int M1,
+                                                         long long K, long long kchunk, int nsplit, int tiles_n,
+                                                         double2 *__restrict__ work) {
+  __shared__ double2 sA[2][kBM3 * kLdsRow];
+  __shared__ double2 sB[2][kBN3 * kLdsRow];
+  const int bid = blockIdx.x;
+  const int chunk = bid % nsplit;
+  const int tile = bid / nsplit;
+  const int tm = tile / tiles_n, tn = tile % tiles_n;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int r16 = lane & 15, ks = lane >> 4;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+
+  // global -> LDS: A row t/4 elements 4(t%4)..+3, B row t/8 elements 2(t%8)..+1
+  const int arow = tid >> 2, ae = (tid & 3) * 4;
+  const int brow = tid >> 3, be = (tid & 7) * 2;
+  int ga = tm * kBM3 + arow;
+  if (ga >= M0) ga = M0 - 1;  // clamped rows feed outputs that are never stored
+  int gb = tn * kBN3 + brow;
+  if (gb >= M1) gb = M1 - 1;
+  const double2 *pa = A + (size_t)ga * lda;
+  const double2 *pb = B + (size_t)gb * ldb;
+
+  const long long k0 = (long long)chunk * kchunk;
+  const long long k1 = (k0 + kchunk < K) ? k0 + kchunk : K;
+  const int nstage = k1 > k0 ? (int)((k1 - k0 + kKC - 1) / kKC) : 0;
+
+  double2 ra[4], rb[2];
+  auto gload = [&](long long kb) {
+    if (kb + kKC <= k1) {  // whole stage in range (uniform over the block)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) ra[s] = pa[kb + ae + s];
+#pragma unroll
+      for (int s = 0; s < 2; ++s) rb[s] = pb[kb + be + s];
+      return;
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const long long kk = kb + ae + s;
+      ra[s] = kk < k1 ? pa[kk] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const long long kk = kb + be + s;
+      rb[s] = kk < k1 ? pb[kk] : make_double2(0.0, 0.0);
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) sA[buf][arow * kLdsRow + ae + s] = ra[s];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) sB[buf][brow * kLdsRow + be + s] = rb[s];
+  };
+
+  v4d p1[2][2], p2[2][2], p3[2][2];
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) p1[ti][tj] = p2[ti][tj] = p3[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  if (nstage > 0) {
+    gload(k0);
+    lstore(0);
+  }
+  __syncthreads();
+  for (int st = 0; st < nstage; ++st) {
+    const int cur = st & 1;
+    const bool more = st + 1 < nstage;
+    if (more) gload(k0 + (long long)(st + 1) * kKC);
+    mma3_stage(sA[cur], sB[cur], wm + r16, wn + r16, ks, p1, p2, p3);
+    if (more) lstore(cur ^ 1);
+    __syncthreads();
+  }
+
+  // D layout of v_mfma_f64_16x16x4: element r of lane l is D[l/16 + 4*r][l % 16].
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = tm * kBM3 + wm + ti * 16 + ks + 4 * r;
+        const int nn = tn * kBN3 + wn + tj * 16 + r16;
+        if (m < M0 && nn < M1) {
+          const double a1 = p1[ti][tj][r], a2 = p2[ti][tj][r];
+          work[((size_t)chunk * M0 + m) * M1 + nn] = make_double2(a1 + a2, p3[ti][tj][r] - a1 + a2);
+        }
+      }
+}
+
 __global__ void k_contract_reduce(const double2 *__restrict__ work, int nsplit, int M0, int M1,
                                   double2 *__restrict__ out, int ldo, int accumulate) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -266,20 +400,29 @@ __global__ void k_contract_reduce(const double2 *__restrict__ work, int nsplit, 
   *o = make_double2(sx, sy);
 }
 
-// Kernel choice: the LDS-tiled 128 x 128 kernel for tables of at least
-// 128 x 128 (LQ_CONTRACT=wave forces the register-only 64 x 64 kernel).
-bool use_lds(int M0, int M1) {
-  static const bool force_wave = [] {
+// Kernel choice for tables of at least 128 x 128: the three-multiplication
+// kernel (LQ_CONTRACT=4m selects the four-multiplication LDS kernel); smaller
+// tables, or LQ_CONTRACT=wave, use the register-only 64 x 64 kernel.
+enum ContractKind { kWave = 0, kLds4m = 1, kLds3m = 2 };
+
+int contract_kind(int M0, int M1) {
+  static const int forced = [] {
     const char *e = getenv("LQ_CONTRACT");
-    return e && strcmp(e, "wave") == 0;
+    if (e && strcmp(e, "wave") == 0) return (int)kWave;
+    if (e && strcmp(e, "4m") == 0) return (int)kLds4m;
+    return -1;
   }();
-  return !force_wave && M0 >= kBT && M1 >= kBT;
+  if (M0 < kBT || M1 < kBT) return kWave;
+  return forced >= 0 ? forced : kLds3m;
 }
 
+bool use_lds(int M0, int M1) { return contract_kind(M0, M1) != kWave; }
+
 int choose_nsplit(int M0, int M1, long long K) {
-  const int bt = use_lds(M0, M1) ? kBT : 64;
-  const int target = use_lds(M0, M1) ? 512 : 2048;  // blocks in flight
-  const long long tiles = (long long)((M0 + bt - 1) / bt) * ((M1 + bt - 1) / bt);
+  const int kind = contract_kind(M0, M1);
+  const int bm = kind == kWave ? 64 : kBT, bn = kind == kLds3m ? kBN3 : bm;
+  const int target = kind == kWave ? 2048 : 512;  // blocks in flight
+  const long long tiles = (long long)((M0 + bm - 1) / bm) * ((M1 + bn - 1) / bn);
   long long ns = (target + tiles - 1) / tiles;
   ns = ((ns + 7) / 8) * 8;  // whole XCD groups
   if (ns > 256) ns = 256;
@@ -304,17 +447,20 @@ int launch_contract_f64(hipStream_t st, const double2 *A, long long lda, int M0,
     set_error("launch_contract_f64: bad sizes M0=%d M1=%d K=%lld nsplit=%d", M0, M1, K, nsplit);
     return LQ_ERR_ARG;
   }
-  const bool lds = use_lds(M0, M1);
-  const int bt = lds ? kBT : 64;
+  const int kind = contract_kind(M0, M1);
+  const int bm = kind == kWave ? 64 : kBT, bn = kind == kLds3m ? kBN3 : bm;
   long long kchunk = (K + nsplit - 1) / nsplit;
   kchunk = ((kchunk + 15) / 16) * 16;
-  const int tiles_m = (M0 + bt - 1) / bt, tiles_n = (M1 + bt - 1) / bt;
+  const int tiles_m = (M0 + bm - 1) / bm, tiles_n = (M1 + bn - 1) / bn;
   const long long blocks = (long long)tiles_m * tiles_n * nsplit;
   if (blocks > 0x7fffffffLL) {
     set_error("launch_contract_f64: grid too large");
     return LQ_ERR_ARG;
   }
-  if (lds)
+  if (kind == kLds3m)
+    hipLaunchKernelGGL(k_contract_f64_3m, dim3((unsigned)blocks), dim3(512), 0, st, A, lda, M0, B, ldb, M1, K,
+                       kchunk, nsplit, tiles_n, work);
+  else if (kind == kLds4m)
     hipLaunchKernelGGL(k_contract_f64_lds, dim3((unsigned)blocks), dim3(512), 0, st, A, lda, M0, B, ldb, M1, K,
                        kchunk, nsplit, tiles_n, work);
   else

@@ -60,7 +60,8 @@ def test_spmv_general_fill_and_rescale(ctx, oracle):
     assert rel_inf(G.Multiply(1.0, x), Dg @ x) < 1e-14
 
 
-@pytest.mark.parametrize("ML,MR,K", [(70, 50, 1000), (64, 64, 4096 * 9 + 5), (1, 3, 17), (130, 129, 300)])
+@pytest.mark.parametrize("ML,MR,K", [(70, 50, 1000), (64, 64, 4096 * 9 + 5), (1, 3, 17), (130, 129, 300),
+                                      (256, 200, 4096 * 5 + 3)])
 def test_compute_mom_table(ctx, ML, MR, K):
     import ctypes as C
     from linqt_amd import _lib

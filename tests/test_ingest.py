@@ -1,0 +1,126 @@
+"""File ingestion through the C ABI (host code, no GPU): the native CSR and
+moment readers against the independent numpy readers of ``linqt_amd.io``,
+byte-exact reproduction of files the reference's own writers produced, and
+exact round trips of the 17-digit text and binary forms."""
+import os
+
+import numpy as np
+import pytest
+
+import linqt_amd as lq
+from conftest import GOLDEN
+from linqt_amd import io, lattices
+
+W2S = os.path.join(GOLDEN, "wannier2sparse")
+LC = os.path.join(GOLDEN, "linear_chain")
+
+
+def _same_csr(a, b):
+    assert a.n == b.n and a.nnz == b.nnz
+    assert np.array_equal(a.rowptr, b.rowptr) and np.array_equal(a.col, b.col)
+    assert np.array_equal(a.val.view(np.int64), b.val.view(np.int64))  # bitwise, signed zeros included
+
+
+def test_native_csr_reader_matches_numpy_reader_on_reference_files():
+    for p in (os.path.join(LC, "operators", "linear_chain.HAM.CSR"), os.path.join(LC, "operators", "linear_chain.VX.CSR"),
+              os.path.join(W2S, "linear_chain.VXSY.CSR"), os.path.join(W2S, "linear_chain.TX.CSR")):
+        _same_csr(lq.read_csr_file(p), io.read_csr(p))
+
+
+@pytest.mark.parametrize("name", ["linear_chain.VXSY.CSR", "linear_chain.TX.CSR", "linear_chain.HAM.CSR",
+                                  "linear_chain.VX.CSR"])
+def test_reference_csr_writer_format_byte_for_byte(tmp_path, name):
+    """Files written by hopping_list::save_hopping_list_as_csr (6 significant
+    digits, 'v ' separators, empty lines for nnz = 0) are reproduced exactly
+    from their own parsed values."""
+    src = os.path.join(W2S, name)
+    out = tmp_path / name
+    lq.write_csr_file(out, lq.read_csr_file(src), lq.LQ_FMT_TEXT_REF)
+    assert out.read_bytes() == open(src, "rb").read()
+
+
+def test_csr_exact_text_and_binary_round_trips(tmp_path):
+    H, Vx, _ = lattices.haldane_honeycomb(24, seed=3)
+    # a few awkward values: signed zero, subnormal, huge, many digits
+    val = H.val.copy()
+    val[:4] = [complex(-0.0, 0.0), complex(5e-324, -1e300), complex(1 / 3, -2 / 3), complex(-0.1, 1e-17)]
+    H = io.CSR(H.n, H.rowptr, H.col, val)
+    for fmt, name in ((lq.LQ_FMT_TEXT, "h.CSR"), (lq.LQ_FMT_BINARY, "h.bin")):
+        p = tmp_path / name
+        lq.write_csr_file(p, H, fmt)
+        _same_csr(lq.read_csr_file(p), H)
+    r = io.read_csr(str(tmp_path / "h.CSR"))  # numpy's text-to-float drops the sign of "-0"
+    assert np.array_equal(r.val, H.val) and np.array_equal(r.col, H.col)
+
+
+def test_csr_reader_many_chunks(tmp_path):
+    """A multi-megabyte text file is cut into several chunks; chunk edges fall
+    inside numbers and must not split them."""
+    m = lattices.cubic_anderson(40, W=2.0, seed=5)  # 64000 rows, ~4 MB of text
+    p = tmp_path / "big.CSR"
+    lq.write_csr_file(p, m, lq.LQ_FMT_TEXT)
+    assert os.path.getsize(p) > 3 << 20
+    _same_csr(lq.read_csr_file(p), m)
+    _same_csr(lq.read_csr_file(p), io.read_csr(str(p)))
+
+
+def test_csr_reader_errors(tmp_path):
+    with pytest.raises(lq.LinqtError, match="Can not open"):
+        lq.read_csr_file(tmp_path / "missing.CSR")
+    p = tmp_path / "trunc.CSR"
+    p.write_text("3 2\n1 0 2 0\n0 1\n0 1\n")
+    with pytest.raises(lq.LinqtError, match="truncated"):
+        lq.read_csr_file(p)
+    p.write_text("2 1\n1 x\n0\n0 1 1\n")
+    with pytest.raises(lq.LinqtError, match="unreadable"):
+        lq.read_csr_file(p)
+    p.write_text("2 1\n1 0\n0.5\n0 1 1\n")  # a column must be an integer
+    with pytest.raises(lq.LinqtError, match="unreadable"):
+        lq.read_csr_file(p)
+    p.write_text("+2 1\n+1.5 -0\n1\n0 1 1\n")  # '+' signs are read as `>>` reads them
+    r = lq.read_csr_file(p)
+    assert r.n == 2 and r.val[0] == 1.5 and np.signbit(r.val[0].imag)
+
+
+def test_moment_files_reference_format_byte_for_byte(tmp_path):
+    """The reference-written .chebmom2D files (15 digits, "-0" included) are
+    reproduced exactly from the natively read values, which equal the numpy
+    reader's."""
+    for name in ("NonLEqOpVX-VXlinear_chainKPM_M100x100.chebmom2D", "NonLEqOpVX-VXlinear_chainKPM_M300x300.chebmom2D"):
+        src = os.path.join(LC, name)
+        mu, h = lq.read_mom_file(src)
+        ref, n, W, Ec = io.read_chebmom2d(src)
+        assert h["kind"] == lq.MOM_2D and (h["n"], h["W"], h["Ec"]) == (n, W, Ec)
+        assert np.array_equal(mu, ref)  # by value: numpy's reader drops the sign of "-0"
+        out = tmp_path / name
+        lq.write_mom_file(out, mu, h["n"], h["W"], h["Ec"])
+        assert out.read_bytes() == open(src, "rb").read()
+
+
+def test_moment_files_all_kinds_round_trip(tmp_path):
+    rng = np.random.default_rng(7)
+    mu1 = rng.normal(size=37) + 1j * rng.normal(size=37)
+    mu2 = rng.normal(size=(5, 9)) + 1j * rng.normal(size=(5, 9))
+    cases = [("a.chebmom1D", mu1, lq.MOM_1D, 0.0), ("a.chebmom2D", mu2, lq.MOM_2D, 0.0),
+             ("a.chebmomTD", mu2, lq.MOM_TD, 0.25)]
+    for name, mu, kind, dt in cases:
+        # the 15-digit text agrees with the Python writers byte for byte
+        p = tmp_path / name
+        lq.write_mom_file(p, mu, 1234, 9.2, -0.5, kind=kind, dt=dt)
+        q = tmp_path / ("py_" + name)
+        if kind == lq.MOM_1D:
+            io.write_chebmom1d(str(q), mu, 1234, 9.2, -0.5)
+        elif kind == lq.MOM_2D:
+            io.write_chebmom2d(str(q), mu, 1234, 9.2, -0.5)
+        else:
+            io.write_chebmomtd(str(q), mu, 1234, 9.2, -0.5, dt)
+        assert p.read_bytes() == q.read_bytes(), name
+        # exact forms
+        for fmt, suffix in ((lq.LQ_FMT_TEXT, ""), (lq.LQ_FMT_BINARY, ".bin")):
+            e = tmp_path / ("x_" + name + suffix)
+            lq.write_mom_file(e, mu, 1234, 9.2, -0.5, kind=kind, dt=dt, fmt=fmt)
+            r, h = lq.read_mom_file(e)
+            assert h == {"kind": kind, "n": 1234, "W": 9.2, "Ec": -0.5, "dt": dt}
+            assert np.array_equal(r, mu)
+    with pytest.raises(lq.LinqtError, match="valid"):
+        lq.read_mom_file(tmp_path / "a.txt") if (tmp_path / "a.txt").write_text("1 2 3\n4\n") else None

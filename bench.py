@@ -371,13 +371,25 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     value = moments_per_step / (elapsed / args.steps)
     spmv_gbs = tm["spmv_bytes"] / (tm["spmv_ms"] * 1e-3) / 1e9 if tm["spmv_ms"] > 0 else 0.0
-    ctr_tfs = tm["contract_flops"] / (tm["contract_ms"] * 1e-3) / 1e12 if tm["contract_ms"] > 0 else 0.0
+    # complex-equivalent rate (8 flops per complex multiply-add) and the rate of
+    # the real MFMA flops the FP64 kernel executes: the three-multiplication
+    # kernel (tables of at least 128 x 128) does 6 per complex multiply-add
+    ctr_cplx_tfs = tm["contract_flops"] / (tm["contract_ms"] * 1e-3) / 1e12 if tm["contract_ms"] > 0 else 0.0
+    ctr_mode = os.environ.get("LQ_CONTRACT", "")
+    if ctr_mode == "4m":
+        ctr_kernel, ctr_fpc = "k_contract_f64_lds", 8
+    elif ctr_mode == "wave" or config.get("M", 0) < 128:
+        ctr_kernel, ctr_fpc = "k_contract_f64", 8
+    else:
+        ctr_kernel, ctr_fpc = "k_contract_f64_3m", 6
+    ctr_tfs = ctr_cplx_tfs * ctr_fpc / 8
     kernels = {
         "spmv": {"ms_per_step": tm["spmv_ms"] / tsteps, "launches_per_step": tm["spmv_launches"] / tsteps,
                  "GB/s": spmv_gbs, "frac_hbm": spmv_gbs / HBM_PEAK_GBS},
         "contract": {"ms_per_step": tm["contract_ms"] / tsteps,
                      "launches_per_step": tm["contract_launches"] / tsteps,
-                     "TFLOP/s": ctr_tfs, "frac_fp64_mfma": ctr_tfs / FP64_MFMA_PEAK_TFS},
+                     "kernel": ctr_kernel, "TFLOP/s": ctr_tfs, "frac_fp64_mfma": ctr_tfs / FP64_MFMA_PEAK_TFS,
+                     "complex_equiv_TFLOP/s": ctr_cplx_tfs},
         "post_ms_per_step": tm["post_ms"] / tsteps,
         "instrumented_step": "one extra step after the timed ones, HIP events around every launch",
     }
@@ -424,11 +436,14 @@ def main():
                     "per_launch": f"{ops:.4g} int8 op (8 M^2 ceil(N R / 4096) 4096 S(S+1)/2)",
                     "avg_launch_ms": avg_ms}
     elif tm["contract_ms"] > tm["spmv_ms"]:
-        per_launch = tm["contract_flops"] / max(tm["contract_launches"], 1)
-        kname = "k_contract_f64_lds" if min(config["M"], config["M"]) >= 128 else "k_contract_f64"
+        per_launch = tm["contract_flops"] / max(tm["contract_launches"], 1) * ctr_fpc / 8
+        kname = ctr_kernel
+        how = ("6*M^2*N*R: three real FP64 MFMA products per complex multiply-add" if ctr_fpc == 6
+               else "8*M^2*N*R: four real FP64 MFMA products per complex multiply-add")
         roofline = {"bound": "mfma", "kernel": kname, "achieved": ctr_tfs, "peak": FP64_MFMA_PEAK_TFS,
                     "unit": "TFLOP/s", "frac": ctr_tfs / FP64_MFMA_PEAK_TFS, "traffic": traffic_tab.get(kname),
-                    "per_launch": f"{per_launch:.4g} flop (8*M^2*N*R)",
+                    "per_launch": f"{per_launch:.4g} flop ({how})",
+                    "complex_equiv_TFLOP/s": ctr_cplx_tfs,
                     "avg_launch_ms": tm["contract_ms"] / max(tm["contract_launches"], 1)}
     else:
         per_launch = tm["spmv_bytes"] / max(tm["spmv_launches"], 1)

@@ -115,6 +115,23 @@ int lq_csr_destroy(lq_csr A);
 int lq_csr_rescale(lq_csr A, double a_re, double a_im, double b_re, double b_im);
 /* n, completed nnz, stored (padded) entries. */
 int lq_csr_info(lq_csr A, int *n, long long *nnz, long long *nnz_stored);
+/* The operator's stored entries (upper triangle with the diagonal for
+ * LQ_FILL_HERMITIAN_UPPER, all for LQ_FILL_GENERAL) as zero-based CSR, the
+ * arrays a reference CSR file holds; after lq_csr_rescale the rescaled ones.
+ * rowptr == NULL: *nnz only; otherwise arrays of n+1, nnz, 2*nnz entries. */
+int lq_csr_export(lq_csr A, long long *nnz, int *rowptr, int *col, double *val);
+/* Replaces wannier2sparse's hopping_list::wrap_in_supercell +
+ * save_hopping_list_as_csr (utilities/wannier2sparse/src/hopping_list.cpp:
+ * 156-285) and the Hermitian completion of the CSR file it writes: the
+ * operator of a unit-cell hopping list on a dims[0] x dims[1] x dims[2]
+ * supercell, generated on the device (LQ_FILL_HERMITIAN_UPPER).  Per hopping:
+ * cell[3] (unit-cell vector), edge[2] (zero-based orbitals from, to), val
+ * (re, im).  Hoppings are taken in the order of the reference's multimap
+ * (tag string "c0 c1 c2 e0 e1 "), those with |v| <= 1e-10 mean|v| and those
+ * below the diagonal are dropped, duplicates are summed in that order; the
+ * stored entries equal the file's up to its 6-digit text rounding. */
+int lq_csr_from_hoppings(lq_ctx ctx, int nwann, int nhop, const int *cell, const int *edge,
+                         const double *val, const int *dims, lq_csr *out);
 
 /* Replaces SparseMatrixType::Multiply(a, x, b, y) (src/mkl_sparse_matrix.cpp:56-69)
  * and BatchMultiply (:102-106): y <- a*A*x + b*y for k interleaved vectors.

@@ -143,6 +143,41 @@ class SparseMatrix:
         """``SparseMatrixBuilder::BuildOPFromCSRFile`` (text or binary CSR, parsed natively)."""
         return cls.from_csr(ctx, read_csr_file(path), id=id)
 
+    @classmethod
+    def from_hoppings(cls, ctx: Context, nwann: int, cell, edge, val, dims, id: str = ""):
+        """wannier2sparse's ``wrap_in_supercell`` + ``save_hopping_list_as_csr``
+        (utilities/wannier2sparse/src/hopping_list.cpp:156-285) generated on the
+        device: the Hermitian-upper operator of a unit-cell hopping list on a
+        ``dims`` supercell.  cell: (nhop, 3) ints, edge: (nhop, 2) zero-based
+        orbitals (from, to), val: (nhop,) complex."""
+        cell = np.ascontiguousarray(np.asarray(cell, dtype=np.int32).reshape(-1, 3))
+        edge = np.ascontiguousarray(np.asarray(edge, dtype=np.int32).reshape(-1, 2))
+        v = np.ascontiguousarray(np.asarray(val, dtype=np.complex128).reshape(-1))
+        nh = v.shape[0]
+        if nh == 0:
+            cell, edge, v = np.zeros((1, 3), np.int32), np.zeros((1, 2), np.int32), np.zeros(1, np.complex128)
+        d = np.ascontiguousarray(np.asarray(dims, dtype=np.int32).reshape(3))
+        self = cls.__new__(cls)
+        self.ctx, self.id = ctx, id
+        h = C.c_void_p()
+        _lib.call("lq_csr_from_hoppings", ctx.h, int(nwann), nh, _ip(cell), _ip(edge), _dp(v.view(np.float64)),
+                  _ip(d), C.byref(h))
+        self.h = h
+        self.n = self.info()["n"]
+        return self
+
+    def export(self):
+        """The stored entries (what a reference CSR file holds) as ``io.CSR``."""
+        from .io import CSR
+
+        nnz = C.c_longlong(0)
+        _lib.call("lq_csr_export", self.h, C.byref(nnz), None, None, None)
+        rowptr = np.zeros(self.n + 1, np.int32)
+        col = np.zeros(max(nnz.value, 1), np.int32)
+        val = np.zeros(max(nnz.value, 1), np.complex128)
+        _lib.call("lq_csr_export", self.h, C.byref(nnz), _ip(rowptr), _ip(col), _dp(val.view(np.float64)))
+        return CSR(self.n, rowptr, col[: nnz.value], val[: nnz.value])
+
     def isIdentity(self) -> bool:
         return self.id == "1"
 

@@ -60,7 +60,7 @@ def build_cxx(verbose: bool = False) -> None:
     host-only code over the C ABI."""
     cxx = os.environ.get("CXX", "g++")
     tmp = CXX_LIB + ".tmp"
-    _run([cxx, *CXXFLAGS, "-shared", os.path.join(HERE, "cpp", "linqt_facade.cpp"), "-o", tmp,
+    _run([cxx, *CXXFLAGS, "-shared", *sorted(glob.glob(os.path.join(HERE, "cpp", "*.cpp"))), "-o", tmp,
           "-L", HERE, "-llinqt_amd", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, CXX_LIB)
     os.makedirs(BIN, exist_ok=True)

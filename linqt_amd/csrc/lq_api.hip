@@ -472,6 +472,23 @@ int spmm(lq_ctx ctx, lq_csr A, double2 alpha, double2 beta, const VecSet *sets, 
 
 }  // namespace
 
+// A new operator around a host CSR already filled in (rowptr / col / val /
+// kind, rows sorted, columns ascending): uploads its device forms.
+int csr_adopt(lq_ctx ctx, lq_csr A, lq_csr *out) {
+  static std::atomic<unsigned long long> next_uid{1};
+  A->ctx = ctx;
+  A->uid = next_uid++;
+  DeviceGuard g(ctx->device);
+  const int rc = upload_sell(A);
+  if (rc != LQ_OK) {
+    free_dev(A->dev);
+    delete A;
+    return rc;
+  }
+  *out = A;
+  return LQ_OK;
+}
+
 bool is_device_ptr(const void *p, int device) {
   hipPointerAttribute_t attr;
   if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
@@ -900,21 +917,35 @@ int lq_csr_create(lq_ctx ctx, int n, int nnz, const int *rowptr, const int *col,
       }
     }
   }
-  DeviceGuard g(ctx->device);
-  static std::atomic<unsigned long long> next_uid{1};
   lq_csr A = new lq_csr_s();
-  A->ctx = ctx;
-  A->uid = next_uid++;
   A->n = n;
   A->fill = fill;
   set_entries(A, e);
-  int rc = upload_sell(A);
-  if (rc != LQ_OK) {
-    free_dev(A->dev);
-    delete A;
-    return rc;
+  return csr_adopt(ctx, A, out);
+}
+
+int lq_csr_export(lq_csr A, long long *nnz, int *rowptr, int *col, double *val) {
+  LQ_TRY(check_op(A, "lq_csr_export"));
+  if (!nnz || (rowptr && (!col || !val))) {
+    set_error("lq_csr_export: null argument");
+    return LQ_ERR_ARG;
   }
-  *out = A;
+  long long m = 0;
+  for (signed char k : A->kind) m += k != 1;
+  *nnz = m;
+  if (!rowptr) return LQ_OK;
+  long long q = 0;
+  rowptr[0] = 0;
+  for (int i = 0; i < A->n; ++i) {
+    for (int p = A->rowptr[i]; p < A->rowptr[i + 1]; ++p) {
+      if (A->kind[p] == 1) continue;
+      col[q] = A->col[p];
+      val[2 * q] = A->val[p].real();
+      val[2 * q + 1] = A->val[p].imag();
+      ++q;
+    }
+    rowptr[i + 1] = (int)q;
+  }
   return LQ_OK;
 }
 

@@ -178,6 +178,14 @@ __global__ __launch_bounds__(512) void k_contract_f64_lds(const double2 *__restr
 
   double2 ra[4], rb[4];
   auto gload = [&](long long kb) {
+    if (kb + kKC <= k1) {  // whole stage in range (uniform over the block)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        ra[s] = pa[kb + le + s];
+        rb[s] = pb[kb + le + s];
+      }
+      return;
+    }
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const long long kk = kb + le + s;

@@ -240,6 +240,9 @@ struct lq_csr_s {
 };
 
 namespace lq {
+// Takes ownership of A (host CSR filled in) and uploads it; on error A is freed.
+int csr_adopt(lq_ctx ctx, lq_csr A, lq_csr *out);
+
 // Makes a context's device current for the scope of a call.
 struct DeviceGuard {
   int prev = -1;

@@ -64,6 +64,8 @@ def lib():
             getattr(L, fn).argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
         for fn in ("lqo_greenwood_fftgrid", "lqo_bastin_fftgrid"):
             getattr(L, fn).argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
+        L.lqo_supercell_csr.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _ip, _ip, C.POINTER(C.c_longlong), _ip,
+                                        _ip, _dp]
         L.lqo_omp_threads.restype = C.c_int
         L.lqo_omp_cheb_step.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.lqo_omp_momtable.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp]
@@ -263,6 +265,28 @@ def linear_grid(kind: int, mu: np.ndarray, n: int, W: float, Ec: float, num_div:
     rows = lib().lqo_linear_grid(kind, _d(mu), M0, M1, n, W, Ec, num_div, _d(E), _d(out))
     _check(0 if rows > 0 else -1, "linear_grid")
     return E[:rows], out[:rows]
+
+
+def supercell_csr(nwann: int, cell, edge, val, dims):
+    """wrap_in_supercell + save_hopping_list_as_csr: the stored upper CSR
+    (zero based) of the supercell operator; returns (n, rowptr, col, val)."""
+    cell = np.ascontiguousarray(np.asarray(cell, dtype=np.int32).reshape(-1, 3))
+    edge = np.ascontiguousarray(np.asarray(edge, dtype=np.int32).reshape(-1, 2))
+    v = np.ascontiguousarray(np.asarray(val, dtype=np.complex128).reshape(-1))
+    nh = v.shape[0]
+    if nh == 0:
+        cell, edge, v = np.zeros((1, 3), np.int32), np.zeros((1, 2), np.int32), np.zeros(1, np.complex128)
+    d = np.ascontiguousarray(np.asarray(dims, dtype=np.int32))
+    n, nnz = C.c_int(0), C.c_longlong(0)
+    L = lib()
+    _check(L.lqo_supercell_csr(nwann, nh, _i(cell), _i(edge), _d(v.view(np.float64)), _i(d), C.byref(n),
+                               C.byref(nnz), None, None, None), "supercell_csr")
+    rowptr = np.zeros(n.value + 1, np.int32)
+    col = np.zeros(max(nnz.value, 1), np.int32)
+    out = np.zeros(max(nnz.value, 1), np.complex128)
+    _check(L.lqo_supercell_csr(nwann, nh, _i(cell), _i(edge), _d(v.view(np.float64)), _i(d), C.byref(n),
+                               C.byref(nnz), _i(rowptr), _i(col), _d(out.view(np.float64))), "supercell_csr")
+    return n.value, rowptr, col[: nnz.value], out[: nnz.value]
 
 
 def omp_threads() -> int:

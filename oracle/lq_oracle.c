@@ -11,6 +11,7 @@
 
 #include <complex.h>
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -1187,3 +1188,113 @@ int lqo_omp_spectral_moments(const lqo_op *Hc, int n, const double *phi, int R, 
   free(c1);
   return 0;
 }
+
+/* ======================================================================== */
+/* Supercell operators: hopping_list::wrap_in_supercell +                   */
+/* save_hopping_list_as_csr (utilities/wannier2sparse/src/hopping_list.cpp).*/
+/* ======================================================================== */
+
+typedef struct {
+  char tag[80];
+  int idx;
+} sc_tag;
+
+/* std::multimap<string,...> order: by tag string, equal tags in insertion order */
+static int sc_tag_cmp(const void *a, const void *b) {
+  const sc_tag *x = (const sc_tag *)a, *y = (const sc_tag *)b;
+  int c = strcmp(x->tag, y->tag);
+  return c ? c : (x->idx > y->idx) - (x->idx < y->idx);
+}
+
+typedef struct {
+  int row, col;
+  long long seq;
+  double re, im;
+} sc_trip;
+
+static int sc_trip_cmp(const void *a, const void *b) {
+  const sc_trip *x = (const sc_trip *)a, *y = (const sc_trip *)b;
+  if (x->row != y->row) return x->row < y->row ? -1 : 1;
+  if (x->col != y->col) return x->col < y->col ? -1 : 1;
+  return (x->seq > y->seq) - (x->seq < y->seq);
+}
+
+/* index_aliasing (utilities/wannier2sparse/include/hopping_list_util.hpp:11-15) */
+static int sc_alias(const int *c, const int *b) {
+  return (((c[2] + b[2]) % b[2]) * b[1] + (c[1] + b[1]) % b[1]) * b[0] + (c[0] + b[0]) % b[0];
+}
+
+int lqo_supercell_csr(int nwann, int nhop, const int *cell, const int *edge, const double *val,
+                      const int *dims, int *n, long long *nnz, int *rowptr, int *col, double *out_val) {
+  if (nwann <= 0 || nhop < 0 || !dims || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0 || !n || !nnz) return -1;
+  const long long ncell = (long long)dims[0] * dims[1] * dims[2];
+  if (ncell * nwann > 0x7ffffffeLL) return -1;
+  *n = (int)(ncell * nwann);
+  sc_tag *tags = (sc_tag *)malloc(sizeof(sc_tag) * (nhop > 0 ? nhop : 1));
+  double avg = 0.0;
+  for (int h = 0; h < nhop; ++h) {
+    if (edge[2 * h] < 0 || edge[2 * h] >= nwann || edge[2 * h + 1] < 0 || edge[2 * h + 1] >= nwann) {
+      free(tags);
+      return -1;
+    }
+    snprintf(tags[h].tag, sizeof(tags[h].tag), "%d %d %d %d %d ", cell[3 * h], cell[3 * h + 1], cell[3 * h + 2],
+             edge[2 * h], edge[2 * h + 1]);
+    tags[h].idx = h;
+  }
+  qsort(tags, (size_t)nhop, sizeof(sc_tag), sc_tag_cmp);
+  for (int q = 0; q < nhop; ++q) {
+    const int h = tags[q].idx;
+    avg += sqrt(val[2 * h] * val[2 * h] + val[2 * h + 1] * val[2 * h + 1]);
+  }
+  avg /= nhop; /* NaN for an empty list: then nothing passes the filter */
+  sc_trip *t = (sc_trip *)malloc(sizeof(sc_trip) * (size_t)(ncell * nhop > 0 ? ncell * nhop : 1));
+  long long nt = 0, seq = 0;
+  for (int q = 0; q < nhop; ++q) {
+    const int h = tags[q].idx;
+    const double re = val[2 * h], im = val[2 * h + 1];
+    const int keep = sqrt(re * re + im * im) / avg > 1e-10;
+    int s[3];
+    for (s[2] = 0; s[2] < dims[2]; ++s[2])
+      for (s[1] = 0; s[1] < dims[1]; ++s[1])
+        for (s[0] = 0; s[0] < dims[0]; ++s[0], ++seq) {
+          int c[3];
+          for (int d = 0; d < 3; ++d) c[d] = (cell[3 * h + d] + s[d] + dims[d]) % dims[d];
+          /* the reference wraps with one + dims: keep the same arithmetic, then fold */
+          for (int d = 0; d < 3; ++d) c[d] = ((c[d] % dims[d]) + dims[d]) % dims[d];
+          const int row = edge[2 * h] + sc_alias(s, dims) * nwann;
+          const int cl = edge[2 * h + 1] + sc_alias(c, dims) * nwann;
+          if (!keep || cl < row) continue;
+          t[nt].row = row;
+          t[nt].col = cl;
+          t[nt].seq = seq;
+          t[nt].re = re;
+          t[nt].im = im;
+          ++nt;
+        }
+  }
+  free(tags);
+  qsort(t, (size_t)nt, sizeof(sc_trip), sc_trip_cmp);
+  long long m = 0;
+  for (long long q = 0; q < nt; ++q) {
+    if (m > 0 && t[m - 1].row == t[q].row && t[m - 1].col == t[q].col) {
+      t[m - 1].re += t[q].re;
+      t[m - 1].im += t[q].im;
+    } else {
+      t[m++] = t[q];
+    }
+  }
+  *nnz = m;
+  if (rowptr) {
+    for (int i = 0; i <= *n; ++i) rowptr[i] = 0;
+    for (long long q = 0; q < m; ++q) {
+      rowptr[t[q].row + 1]++;
+      col[q] = t[q].col;
+      out_val[2 * q] = t[q].re;
+      out_val[2 * q + 1] = t[q].im;
+    }
+    for (int i = 0; i < *n; ++i) rowptr[i + 1] += rowptr[i];
+  }
+  free(t);
+  return 0;
+}
+

@@ -14,6 +14,7 @@
  *                                         include/Kubo_solver_FFT.hpp:99-107
  *   - Kubo_solver_FFT (Greenwood/Bastin). src/Kubo_solver_FFT.cpp, src/Greenwood_FFTs.cpp, src/Bastin_FFTs.cpp,
  *                                         src/Kubo_solver_FFT_postProcess.cpp
+ *   - supercell operators ............... utilities/wannier2sparse/src/hopping_list.cpp:156-285
  *   - FFT-grid post-processors .......... src/spectralFunctionFromChebmom_FFTgrid.cpp:35-73,
  *                                         src/kuboGreenwoodFromChebmom_FFTgrid.cpp:35-76,
  *                                         src/kuboBastinFromChebmom_FFTgrid.cpp:57-106
@@ -124,6 +125,21 @@ int lqo_linear_kernel(int kind, const double *mu, int M0, int M1, int n, double 
                       const double *x, double *kern);
 int lqo_linear_grid(int kind, const double *mu, int M0, int M1, int n, double W, double Ec,
                     int num_div, double *E_out, double *out);
+
+/* ---- supercell operators (wannier2sparse) ------------------------------ */
+/* hopping_list::wrap_in_supercell + save_hopping_list_as_csr
+ * (utilities/wannier2sparse/src/hopping_list.cpp:156-225, 227-285) on a
+ * unit-cell list of constant hoppings: cell (3 ints), edge (from, to; zero
+ * based orbitals), val (complex) per hopping; dims = supercell size.  The
+ * list is first put in the order of the reference's multimap (the tag
+ * string "c0 c1 c2 e0 e1 ", equal tags in input order), every hopping is
+ * replicated over the supercell shifts (s2, s1, s0 loops), hoppings with
+ * |v| / mean|v| <= 1e-10 or to < from are dropped, and duplicates are summed
+ * in that order (Eigen setFromTriplets).  Output: the stored upper CSR.
+ * Call with rowptr == NULL for *n and *nnz; the mean is over the unit list
+ * (every hopping appears Ncells times in the supercell). */
+int lqo_supercell_csr(int nwann, int nhop, const int *cell, const int *edge, const double *val,
+                      const int *dims, int *n, long long *nnz, int *rowptr, int *col, double *out_val);
 
 /* ---- OpenMP CPU baseline (same algorithm, threads over rows/pairs) ----- */
 int  lqo_omp_threads(void);

@@ -478,3 +478,82 @@ def test_jackson_td_and_time_post(oracle):
     x = np.array([(EF - Ec) / (W / 2)])
     for t in range(7):
         assert abs(o[t] - oracle.linear_kernel(0, td[:, t].copy(), n, W, x)[0]) < 1e-12 * abs(o[t]) + 1e-14
+
+
+# ---- supercell operators (wannier2sparse wrap + save) -------------------------
+def _hermitian_hoppings(rng, W, nh, reach):
+    cell = rng.integers(-reach, reach + 1, size=(nh, 3))
+    edge = rng.integers(0, W, size=(nh, 2))
+    val = rng.normal(size=nh) + 1j * rng.normal(size=nh)
+    cell = np.concatenate([cell, -cell])
+    edge = np.concatenate([edge, edge[:, ::-1]])
+    val = np.concatenate([val, val.conj()])
+    return cell, edge, val
+
+
+def test_supercell_matches_reference_wrap_fixture(oracle):
+    """utilities/wannier2sparse/test/model_from_files/model_from_files.cpp:
+    "WRAP IN SUPERCELL FINAL TEST" -- a two-orbital chain wrapped on a
+    2 x 2 x 1 supercell must give exactly the 24 listed hoppings; the CSR
+    file keeps those with to >= from."""
+    hv = 1.0 + 1.0j
+    cell = [(0, 0, 0), (-1, 0, 0), (0, -1, 0), (0, 0, 0), (1, 0, 0), (0, 1, 0)]
+    edge = [(0, 1), (0, 1), (0, 1), (1, 0), (1, 0), (1, 0)]
+    val = [hv] * 3 + [np.conj(hv)] * 3
+    listed = {(0, 1): hv, (0, 3): hv, (0, 5): hv, (1, 0): np.conj(hv), (1, 4): np.conj(hv), (1, 2): np.conj(hv),
+              (2, 1): hv, (2, 7): hv, (2, 3): hv, (3, 2): np.conj(hv), (3, 6): np.conj(hv), (3, 0): np.conj(hv),
+              (4, 5): hv, (4, 1): hv, (4, 7): hv, (5, 4): np.conj(hv), (5, 0): np.conj(hv), (5, 6): np.conj(hv),
+              (6, 5): hv, (6, 7): hv, (6, 3): hv, (7, 6): np.conj(hv), (7, 2): np.conj(hv), (7, 4): np.conj(hv)}
+    n, rowptr, col, v = oracle.supercell_csr(2, cell, edge, val, (2, 2, 1))
+    assert n == 8
+    got = {(i, int(col[q])): v[q] for i in range(n) for q in range(rowptr[i], rowptr[i + 1])}
+    assert got == {k: x for k, x in listed.items() if k[1] >= k[0]}
+
+
+@pytest.mark.parametrize("dims", [(3, 2, 1), (1, 1, 1), (2, 2, 2), (4, 1, 3), (5, 3, 1)])
+def test_supercell_spectrum_is_bloch_union(oracle, dims):
+    """The completed supercell operator of a Hermitian hopping list has the
+    spectrum of the Bloch Hamiltonians H(k)_{e0,e1} = sum v e^{i k.c} over the
+    supercell's k grid -- also when hoppings wrap onto each other (dims of 1
+    and 2 with reach 2)."""
+    rng = np.random.default_rng(sum(dims))
+    W = 3
+    cell, edge, val = _hermitian_hoppings(rng, W, 14, 2)
+    n, rowptr, col, v = oracle.supercell_csr(W, cell, edge, val, dims)
+    ev = np.linalg.eigvalsh(dense_herm(n, rowptr, col, v))
+    ks = []
+    for m in np.ndindex(*dims):
+        k = 2 * np.pi * np.array(m) / np.array(dims)
+        Hk = np.zeros((W, W), dtype=complex)
+        for c, e, x in zip(cell, edge, val):
+            Hk[e[0], e[1]] += x * np.exp(1j * k @ c)
+        ks.append(np.linalg.eigvalsh(Hk))
+    assert np.allclose(ev, np.sort(np.concatenate(ks)), atol=1e-10)
+
+
+def test_supercell_duplicates_sum_in_tag_string_order(oracle):
+    """Wrapped duplicates are summed in the order of the reference's
+    multimap<string> (tag "c0 c1 c2 e0 e1 "): "10 ..." < "2 ..." < "9 ...",
+    so ((1 + -1e16) + 1e16) = 0, where numeric cell order would give 1."""
+    cell = [(9, 0, 0), (10, 0, 0), (2, 0, 0)]
+    edge = [(0, 0)] * 3
+    val = [1e16, 1.0, -1e16]
+    n, rowptr, col, v = oracle.supercell_csr(1, cell, edge, val, (1, 1, 1))
+    assert n == 1 and list(col) == [0] and v[0] == 0.0
+    # input order does not matter, only the tags
+    n, rowptr, col, v = oracle.supercell_csr(1, cell[::-1], edge, val[::-1], (1, 1, 1))
+    assert v[0] == 0.0
+
+
+def test_supercell_filters(oracle):
+    """|v| / mean|v| <= 1e-10 is dropped before summing; all-zero and empty
+    lists give an operator with no entries (the reference's "2 0" files)."""
+    cell = [(0, 0, 0), (1, 0, 0), (1, 0, 0)]
+    edge = [(0, 0), (0, 0), (0, 0)]
+    n, rowptr, col, v = oracle.supercell_csr(1, cell, edge, [1.0, 1e-12, 2.0], (3, 1, 1))
+    d = dense_herm(n, rowptr, col, v)
+    assert d[0, 1] == 2.0 and d[0, 0] == 1.0  # the 1e-12 hopping is gone, not added
+    n, rowptr, col, v = oracle.supercell_csr(2, [(1, 0, 0)], [(0, 1)], [0.0], (2, 1, 1))
+    assert n == 4 and col.size == 0 and list(rowptr) == [0] * 5
+    n, rowptr, col, v = oracle.supercell_csr(2, np.zeros((0, 3)), np.zeros((0, 2)), [], (2, 1, 1))
+    assert n == 4 and col.size == 0

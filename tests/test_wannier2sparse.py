@@ -1,0 +1,183 @@
+"""The wannier2sparse tool (host model construction in C++, supercell wrap and
+CSR on the GPU) against the reference's own fixtures and the algebra its
+operators obey.
+
+* HAM / VX / VY of the two-orbital graphene model: byte-identical to the
+  files the reference's writer produces from the hopping list its own parser
+  test lists (utilities/wannier2sparse/test/model_from_files/
+  model_from_files.cpp, "GRAPHENE HR TEST"), wrapped by the oracle.
+* Velocities satisfy V = -i [H, X] bond by bond (tbmodel.cpp:103-135).
+* Spinful graphene + Rashba + exchange (graphene_sot fixtures): spin
+  densities are the reference's spin matrices (SX = -sigma_x and SY = -sigma_y
+  after the Hermitian completion of their upper triangle, SZ = sigma_z), spin
+  velocities are {V, sigma}/2 with the Pauli matrices, torques are
+  (u x J).sigma with J the on-site exchange field (tbmodel.cpp:140-460).
+Files hold 6 significant digits, so the algebraic checks carry a 1e-5 relative
+tolerance."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import linqt_amd as lq
+from conftest import GOLDEN, ROOT, dense_herm
+from linqt_amd import io
+
+BIN = os.path.join(ROOT, "linqt_amd", "bin")
+W2S = os.path.join(GOLDEN, "wannier2sparse")
+
+
+def _run(args, cwd):
+    print("[case] wannier2sparse", *args, flush=True)
+    r = subprocess.run([os.path.join(BIN, "wannier2sparse"), *map(str, args)], cwd=cwd, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r
+
+
+def test_usage_without_gpu():
+    r = subprocess.run([os.path.join(BIN, "wannier2sparse")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "LABEL Dim0 Dim1 Dim2" in r.stdout
+    r = subprocess.run([os.path.join(BIN, "wannier2sparse"), "help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+
+
+# the hoppings the reference's parser test expects from graphene_hr.dat (the
+# zero on-site lines are dropped by the 1e-10 relative filter); orbitals zero based
+GRAPHENE = [((0, 0, 0), (0, 1), 1.0 + 0.12j), ((-1, 0, 0), (0, 1), 1.0 + 0.12j), ((0, -1, 0), (0, 1), 1.0 + 0.12j),
+            ((0, 0, 0), (1, 0), 1.0 + 0.21j), ((1, 0, 0), (1, 0), 1.0 + 0.21j), ((0, 1, 0), (1, 0), 1.0 + 0.21j),
+            ((1, 0, 0), (0, 0), 1.0), ((-1, 0, 0), (0, 0), 2.0), ((0, 1, 0), (0, 0), 3.0),
+            ((0, -1, 0), (0, 0), 4.123456 + 0.1j), ((-1, 1, 0), (0, 0), 5.0 - 0.2j), ((1, -1, 0), (0, 0), 1.0 + 1.0j),
+            ((1, 0, 0), (1, 1), 1.0), ((-1, 0, 0), (1, 1), 2.0), ((0, 1, 0), (1, 1), 3.0),
+            ((0, -1, 0), (1, 1), 4.123456 - 0.1j), ((-1, 1, 0), (1, 1), 4.123456 - 0.1j),
+            ((1, -1, 0), (1, 1), 1.0 - 1.0j)]
+LAT = np.array([[1.0, 0.0, 0.0], [0.5, np.sqrt(3) / 2, 0.0], [0.0, 0.0, 1.0]])
+POS = np.array([[0.0, 0.0, 0.0], [0.5, np.sqrt(3) / 6, 0.0]])
+
+
+def _expected_file(tmp_path, oracle, name, hops, dims, W):
+    cell = [h[0] for h in hops]
+    edge = [h[1] for h in hops]
+    val = [h[2] for h in hops]
+    n, rowptr, col, v = oracle.supercell_csr(W, cell, edge, val, dims)
+    p = tmp_path / ("expected_" + name)
+    lq.write_csr_file(p, io.CSR(n, rowptr, col, v), lq.LQ_FMT_TEXT_REF)
+    return p.read_bytes()
+
+
+def _graphene_dir(tmp_path):
+    shutil.copy(os.path.join(W2S, "graphene_hr.dat"), tmp_path / "graphene_hr.dat")
+    with open(tmp_path / "graphene.uc", "w") as f:
+        for v in LAT:
+            f.write(" ".join(repr(float(x)) for x in v) + "\n")
+    with open(tmp_path / "graphene.xyz", "w") as f:
+        f.write("2\n")
+        for k, p in enumerate(POS):
+            f.write(f"C{k} " + " ".join(repr(float(x)) for x in p) + "\n")
+    return tmp_path
+
+
+@pytest.mark.gpu
+def test_graphene_files_byte_identical(tmp_path, oracle):
+    d = _graphene_dir(tmp_path)
+    dims = (4, 3, 1)
+    _run(["graphene", *dims, "VX", "VY"], d)
+    assert (d / "graphene.HAM.CSR").read_bytes() == _expected_file(tmp_path, oracle, "HAM", GRAPHENE, dims, 2)
+    vol = abs(np.linalg.det(LAT))
+    for dir_, name in ((0, "VX"), (1, "VY")):
+        hops = []
+        for c, e, v in GRAPHENE:
+            dvec = POS[e[1]] - POS[e[0]] + np.array(c) @ LAT
+            s = complex(0.0, -dvec[dir_]) if abs(dvec[dir_]) / vol ** (1 / 3) > 1e-10 else 0j
+            hops.append((c, e, v * s))
+        assert (d / f"graphene.{name}.CSR").read_bytes() == _expected_file(tmp_path, oracle, name, hops, dims, 2)
+
+
+def _dense(path):
+    m = io.read_csr(str(path))
+    return dense_herm(m.n, m.rowptr, m.col, m.val)
+
+
+def _sites(pos, lat, dims, W):
+    """Cartesian position of every supercell orbital (row order of the tool)."""
+    out = np.zeros((W * dims[0] * dims[1] * dims[2], 3))
+    for t2 in range(dims[2]):
+        for t1 in range(dims[1]):
+            for t0 in range(dims[0]):
+                idx = (t2 * dims[1] + t1) * dims[0] + t0
+                out[idx * W:(idx + 1) * W] = pos + np.array([t0, t1, t2]) @ lat
+    return out
+
+
+def _bond(x, lat, dims):
+    """Minimum-image displacement matrices x_b - x_a on the periodic supercell."""
+    D = x[None, :, :] - x[:, None, :]
+    best = D.copy()
+    sup = np.array(dims)[:, None] * lat
+    for k0 in (-1, 0, 1):
+        for k1 in (-1, 0, 1):
+            cand = D + (k0 * sup[0] + k1 * sup[1])
+            better = np.linalg.norm(cand, axis=2) < np.linalg.norm(best, axis=2)
+            best[better] = cand[better]
+    return best
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("label", ["graphene+rashba+M_z", "graphene+rashba+M_x"])
+def test_spinful_operators_obey_their_algebra(tmp_path, label):
+    for ext in (".xyz", ".uc", "_hr.dat"):
+        shutil.copy(os.path.join(W2S, label + ext), tmp_path / (label + ext))
+    dims = (3, 3, 1)
+    _run([label, *dims, "all"], tmp_path)
+    lat = np.loadtxt(tmp_path / (label + ".uc"))
+    xyz = open(tmp_path / (label + ".xyz")).read().split()
+    W = int(xyz[0])
+    pos = np.array(xyz[1:], dtype=object).reshape(W, 4)[:, 1:].astype(float)
+    H = _dense(tmp_path / f"{label}.HAM.CSR")
+    n = H.shape[0]
+    assert n == W * 9 and np.allclose(H, H.conj().T)
+    scale = np.abs(H).max()
+    tol = 1e-5
+    # velocities: V = -i [H, X] bond by bond (the model's hoppings do not reach a periodic image twice)
+    D = _bond(_sites(pos, lat, dims, W), lat, dims)
+    V = {}
+    for k, a in enumerate("XYZ"):
+        V[a] = _dense(tmp_path / f"{label}.V{a}.CSR")
+        ref = -1j * D[:, :, k] * H
+        ref[np.abs(D[:, :, k]) < 1e-10] = 0
+        assert np.abs(V[a] - ref).max() <= tol * scale * max(1.0, np.abs(D).max()), a
+    # Pauli matrices on [orbital block of spin up | spin down] of each cell
+    orb = W // 2
+    one = np.eye(n // W)
+    blk = {"X": np.array([[0, 1], [1, 0]], complex), "Y": np.array([[0, -1j], [1j, 0]]),
+           "Z": np.array([[1, 0], [0, -1]], complex)}
+    sig = {a: np.kron(one, np.kron(m, np.eye(orb))) for a, m in blk.items()}
+    S = {a: _dense(tmp_path / f"{label}.S{a}.CSR") for a in "XYZ"}
+    assert np.allclose(S["X"], -sig["X"], atol=tol) and np.allclose(S["Y"], -sig["Y"], atol=tol)
+    assert np.allclose(S["Z"], sig["Z"], atol=tol)
+    for a in "XYZ":
+        for s in "XYZ":
+            VS = _dense(tmp_path / f"{label}.V{a}S{s}.CSR")
+            ref = 0.5 * (V[a] @ sig[s] + sig[s] @ V[a])
+            assert np.abs(VS - ref).max() <= 2 * tol * max(np.abs(V[a]).max(), 1e-30), (a, s)
+    # torques: T_u = (u x J).sigma, J the exchange field of each orbital's on-site spin block
+    H00 = np.zeros((W, W), complex)
+    for line in open(tmp_path / (label + "_hr.dat")).read().splitlines()[4:]:
+        t = line.split()
+        if len(t) >= 7 and t[:3] == ["0", "0", "0"]:
+            H00[int(t[3]) - 1, int(t[4]) - 1] += complex(float(t[5]), float(t[6]))
+    for a, u in (("X", (1, 0, 0)), ("Y", (0, 1, 0)), ("Z", (0, 0, 1))):
+        T = _dense(tmp_path / f"{label}.T{a}.CSR")
+        ref = np.zeros((W, W), complex)
+        for io_ in range(orb):
+            i, j = io_, orb + io_
+            J00 = 0.5 * (H00[i, i] - H00[j, j])
+            J01 = 0.5 * (H00[i, j] + np.conj(H00[j, i]))
+            J = np.array([J01.real, -J01.imag, J00.real])
+            c = np.cross(u, J)
+            m = c[0] * blk["X"] + c[1] * blk["Y"] + c[2] * blk["Z"]
+            ref[np.ix_([i, j], [i, j])] = m
+        ref_full = np.kron(one, ref)
+        assert np.abs(T - ref_full).max() <= tol * max(np.abs(ref).max(), 1.0), a

@@ -25,6 +25,12 @@ R in {1..16}, drawn from a fixed seed) through the batched engine; one step
 is the whole batch, value = sum of M * R over the jobs / s.  The same jobs
 run one after another through the single-job solver once, for comparison.
 
+`--workload ingest` measures operator ingestion: the Hamiltonian of a clean
+512 x 512 Haldane honeycomb generated on the GPU from its unit-cell hopping
+list (wannier2sparse's wrap + CSR + the Hermitian completion), value = stored
+CSR entries made ready on the GPU per second; the same operator's text file
+through the native reader and the reference's stream reader are timed beside it.
+
 Prints one JSON line.  The roofline object describes the kernel class that
 takes most of the step time, measured with HIP events on the library's own
 stream; cpu_baseline times the CPU restatement of the reference algorithm on
@@ -200,12 +206,139 @@ def cpu_baseline_dos(host_H, M, R, phi):
     }
 
 
+def haldane_hoppings(t=1.0, t2=0.1, mass=0.3):
+    """Unit-cell hopping list (cell, edge, value) of a Haldane honeycomb with
+    a staggered mass: 3 nearest-neighbour bonds A->B, 3 second-neighbour bonds
+    per sublattice with phase pi/2 (i t2 / -i t2), each with its Hermitian
+    partner, plus +-mass on site."""
+    base = [((0, 0, 0), (0, 1), -t), ((-1, 0, 0), (0, 1), -t), ((0, -1, 0), (0, 1), -t),
+            ((1, 0, 0), (0, 0), 1j * t2), ((0, 1, 0), (0, 0), -1j * t2), ((-1, 1, 0), (0, 0), 1j * t2),
+            ((1, 0, 0), (1, 1), -1j * t2), ((0, 1, 0), (1, 1), 1j * t2), ((-1, 1, 0), (1, 1), -1j * t2)]
+    cell, edge, val = [], [], []
+    for c, e, v in base:
+        cell += [c, tuple(-x for x in c)]
+        edge += [e, e[::-1]]
+        val += [v, np.conj(v)]
+    for o, m in ((0, mass), (1, -mass)):
+        cell.append((0, 0, 0))
+        edge.append((o, o))
+        val.append(m)
+    return np.array(cell, np.int32), np.array(edge, np.int32), np.array(val, np.complex128)
+
+
+def run_ingest(args, lq, ctx, log):
+    """Operator ingestion: the Hamiltonian of a 512 x 512 Haldane honeycomb
+    (N = 524288) made ready on the GPU.  One step = lq_csr_from_hoppings
+    (supercell wrap, CSR and Hermitian completion generated on the device,
+    sliced-ELL upload).  Also timed once: the same operator's reference text
+    file (6 digits, as wannier2sparse writes it) through the native parallel
+    reader, its binary form, and the full file -> device operator path; the
+    CPU baseline is the reference's stream reader restated in C."""
+    import tempfile
+
+    import oracle
+
+    L = args.L or 512
+    cell, edge, val = haldane_hoppings()
+    dims = (L, L, 1)
+    A0 = lq.SparseMatrix.from_hoppings(ctx, 2, cell, edge, val, dims)
+    m = A0.export()
+    n = A0.n
+    A0.close()
+    tmpdir = tempfile.mkdtemp(prefix="lq_ingest_")
+    ptxt = os.path.join(tmpdir, "haldane.HAM.CSR")
+    pbin = ptxt + ".bin"
+    lq.write_csr_file(ptxt, m, lq.LQ_FMT_TEXT_REF)
+    lq.write_csr_file(pbin, m, lq.LQ_FMT_BINARY)
+    txt_bytes = os.path.getsize(ptxt)
+    log(f"ingest: N={n} stored entries={m.nnz} text file {txt_bytes / 1e6:.1f} MB")
+
+    def step():
+        A = lq.SparseMatrix.from_hoppings(ctx, 2, cell, edge, val, dims)
+        A.close()
+
+    for w in range(args.warmup):
+        step()
+    ctx.synchronize()
+    lat = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        step()
+        ctx.synchronize()
+        lat.append((time.perf_counter() - t0) * 1e3)
+        log(f"step {k + 1}/{args.steps}: {lat[-1]:.1f} ms")
+    elapsed = time.perf_counter() - t_start
+    # one instrumented generation: HIP events around the count and fill kernels
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    A = lq.SparseMatrix.from_hoppings(ctx, 2, cell, edge, val, dims)
+    ctx.synchronize()
+    tm = ctx.timing()
+    ctx.set_timing(False)
+    completed = A.info()["nnz"]
+    A.close()
+    kern_bytes = 21.0 * completed + 8.0 * n  # count: 4 B per row; fill: col + value + kind per entry, rowptr
+    kern_gbs = kern_bytes / (tm["post_ms"] * 1e-3) / 1e9
+
+    def timed(fn, reps=3):
+        best = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best
+
+    t_txt = timed(lambda: lq.read_csr_file(ptxt))
+    t_bin = timed(lambda: lq.read_csr_file(pbin))
+
+    def load():
+        B = lq.SparseMatrix.from_file(ctx, ptxt)
+        ctx.synchronize()
+        B.close()
+
+    t_load = timed(load)
+    entries = m.nnz
+    ms = elapsed * 1e3 / args.steps
+    out = {"metric": "CSR entries made ready on the GPU per second", "value": entries / (ms * 1e-3),
+           "unit": "entries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "complex128",
+           "data": "synthetic",
+           "latency_ms": {"p50": _pct(lat, 50), "p99": _pct(lat, 99)},
+           "config": {"workload": f"ingest: Haldane honeycomb {L}x{L} cells N={n} (t=1, t2=0.1, mass 0.3), "
+                                  f"{entries} stored entries, {completed} completed",
+                      "N": n, "stored_entries": entries, "completed_entries": completed, "M": 0},
+           "roofline": {"bound": "hbm", "kernel": "k_sc_count + k_sc_fill", "achieved": kern_gbs,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kern_gbs / HBM_PEAK_GBS, "traffic": None,
+                        "per_launch": f"{kern_bytes:.4g} B (21 B per completed entry + 8 B per row)",
+                        "avg_launch_ms": tm["post_ms"]},
+           "phases": {"generate_ms": ms, "kernels_ms": tm["post_ms"],
+                      "text_parse_ms": t_txt * 1e3, "text_MB_per_s": txt_bytes / t_txt / 1e6,
+                      "text_entries_per_s": entries / t_txt, "binary_read_ms": t_bin * 1e3,
+                      "text_file_to_device_operator_ms": t_load * 1e3, "text_file_MB": txt_bytes / 1e6}}
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        oracle.csr_read_stream(ptxt)
+        t_ref = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": entries / t_ref, "unit": "entries/s", "cores": 1, "kind": "port",
+                               "sample": f"the reference's stream CSR reader (Sparse::OPERATOR_FromCSRFile, "
+                                         f"token-by-token stdio) on the same {txt_bytes / 1e6:.1f} MB text file: "
+                                         f"{t_ref:.2f} s"}
+    else:
+        out["cpu_baseline"] = None
+    for p in (ptxt, pbin):
+        os.remove(p)
+    os.rmdir(tmpdir)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("bastin", "dos", "batch", "serve"), default="bastin")
+    ap.add_argument("--workload", choices=("bastin", "dos", "batch", "serve", "ingest"), default="bastin")
     ap.add_argument("--requests", type=int, default=200, help="requests of the serve workload")
     ap.add_argument("--rate", type=float, default=200.0, help="mean arrival rate of the serve workload (req/s)")
     ap.add_argument("--jobs", type=int, default=64, help="instances of the batch workload")
@@ -246,6 +379,10 @@ def main():
         dt = (time.perf_counter() - t0) / reps
         print(json.dumps({"calibrate": "d2d copy", "bytes_read": nbytes, "bytes_written": nbytes, "reps": reps,
                           "ms": dt * 1e3, "GB/s (read+write)": 2 * nbytes / dt / 1e9}), flush=True)
+        return
+
+    if args.workload == "ingest":
+        run_ingest(args, lq, ctx, log)
         return
 
     if args.workload == "bastin":

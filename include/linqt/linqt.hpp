@@ -33,6 +33,9 @@ namespace linqt_amd {
 lq_ctx default_context();
 // Aborts with the C ABI's last error when rc != LQ_OK.
 void check(int rc, const char *what);
+// A moment file (text or binary) through lq_mom_file_read: header and the
+// M0*M1 interleaved complex values.
+lq_mom_header read_moments(const std::string &path, std::vector<double> &mu);
 }  // namespace linqt_amd
 
 // ---------------------------------------------------------------------------
@@ -85,8 +88,12 @@ class SparseMatrixType : public SparseMatrixType_BASE {
   std::vector<std::complex<double> > *vals() { return &vals_; }
 
   lq_csr handle() const { return h_; }
+  // The stored arrays as a general (non-completed) matrix, made on first use:
+  // the matrix the reference's Eigen map of these arrays stands for.
+  lq_csr general_handle();
 
  private:
+  lq_csr gh_ = nullptr;
   void device_vectors(size_t bytes);
   std::vector<int> rows_, cols_;
   std::vector<std::complex<double> > vals_;
@@ -122,6 +129,13 @@ void extract_segment(const std::vector<std::complex<double> > &x, size_t size_x,
                      std::vector<std::complex<double> > &y, size_t size_y);
 void introduce_segment(const std::vector<std::complex<double> > &x, size_t size_x,
                        std::vector<std::complex<double> > &y, size_t size_y, size_t start_y);
+}  // namespace linalg
+class SparseMatrixType;
+namespace linalg {
+// src/mkl_linear_algebra.cpp:100-131: orthogonalized = S^-1 original by
+// BiCGSTAB (tol 1e-4, 2000 iterations) on the GPU (lq_overlap_solve).
+void orthogonalize(SparseMatrixType &S, std::vector<std::complex<double> > &orthogonalized,
+                   std::vector<std::complex<double> > &original);
 }  // namespace linalg
 
 // ---------------------------------------------------------------------------
@@ -297,6 +311,23 @@ class Moments1D : public Moments {
   size_t _numMoms;
 };
 
+// include/chebyshev_moments.hpp:194-218
+class Moments1D_nonOrth : public Moments1D {
+ public:
+  Moments1D_nonOrth(const size_t m0) : Moments1D(m0), S_(nullptr) {}
+  Moments1D_nonOrth(SparseMatrixType &S) : Moments1D(), S_(&S) {}
+  Moments1D_nonOrth(const size_t m0, SparseMatrixType &S) : Moments1D(m0), S_(&S) {}
+  Moments1D_nonOrth(std::string &momfilename, SparseMatrixType &S) : Moments1D(momfilename), S_(&S) {}
+  void set_S(SparseMatrixType &S) { S_ = &S; }
+  SparseMatrixType *S() const { return S_; }
+  void SetInitVectors_nonOrthogonal(vector_t &T0);
+  void SetInitVectors_nonOrthogonal(SparseMatrixType &OP, vector_t &T0);
+  int Iterate_nonOrthogonal();
+
+ private:
+  SparseMatrixType *S_;
+};
+
 class Moments2D : public Moments {
  public:
   Moments2D() : numMoms({0, 0}) {}
@@ -456,6 +487,8 @@ int CorrelationExpansionMoments(const vector_t &PhiR, const vector_t &PhiL, Spar
 int CorrelationExpansionMoments(SparseMatrixType &OPL, SparseMatrixType &OPR, chebyshev::Moments2D &chebMoms,
                                 qstates::generator &gen);
 int SpectralMoments(SparseMatrixType &OP, chebyshev::Moments1D &chebMoms, qstates::generator &gen);
+// src/chebyshev_solver.cpp:203-229: one device solve (lq_spectral_moments_nonorth).
+int SpectralMoments_nonOrth(SparseMatrixType &OP, chebyshev::Moments1D_nonOrth &chebMoms, qstates::generator &gen);
 int TimeDependentCorrelations(SparseMatrixType &OPL, SparseMatrixType &OPR, chebyshev::MomentsTD &chebMoms,
                               qstates::generator &gen);
 

@@ -180,6 +180,25 @@ int lq_kubo_fft(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, const double 
 int lq_kubo_post(lq_ctx ctx, int formula, const double *final_data, int nump, int n, double W,
                  double Ec, double *E, double *sigma, double *integrand);
 
+/* ---- non-orthogonal basis -------------------------------------------------- */
+/* Replaces linalg::orthogonalize (src/mkl_linear_algebra.cpp:100-131):
+ * x = S^-1 b by BiCGSTAB as Eigen runs it (diagonal preconditioner, zero
+ * initial guess, stop at |r| <= tol |b| or after maxit iterations; tol <= 0
+ * and maxit <= 0 select the reference's 1e-4 and 2000).  b, x: distinct
+ * DEVICE vectors of n complex.  The reference maps the overlap's CSR arrays
+ * as a general matrix: create S with LQ_FILL_GENERAL from the file's arrays
+ * to solve with the same matrix.  iterations / error (|r| / |b|) optional. */
+int lq_overlap_solve(lq_csr S, const double *b, double *x, double tol, int maxit, int *iterations,
+                     double *error);
+/* Replaces chebyshev::SpectralMoments_nonOrth (src/chebyshev_solver.cpp:203-229)
+ * with Moments1D_nonOrth::SetInitVectors_nonOrthogonal / Iterate_nonOrthogonal
+ * (src/chebyshev_moments.cpp:63-153): T0 = [OP] S^-1 phi, T1 = S^-1 H T0,
+ * T_{m+1} = 2 S^-1 H T_m - T_{m-1}, every solve as lq_overlap_solve;
+ * mu (M complex) += (2/R)(1/2 for m = 0) <phi|T_m>.  H already rescaled;
+ * OP == NULL is the identity.  iterations: total BiCGSTAB iterations (optional). */
+int lq_spectral_moments_nonorth(lq_csr H, lq_csr OP, lq_csr S, const double *phi, int R, int M, double tol,
+                                int maxit, double *mu, long long *iterations);
+
 /* ---- Jackson kernels and energy-grid post-processors --------------------- */
 /* Moments1D::ApplyJacksonKernel (src/chebyshev_moments1D.cpp:68-87);
  * *M_out receives the truncated count, mu compacted in place. */

@@ -34,7 +34,7 @@ from ._lib import (LQ_ALGO_DIRECT, LQ_ALGO_DOUBLING, LQ_FILL_GENERAL, LQ_FILL_HE
 __all__ = [
     "Context", "SparseMatrix", "DeviceStates", "Batch", "read_csr_file", "write_csr_file", "read_mom_file",
     "write_mom_file", "linear_grid", "time_correlations", "jackson_td",
-    "timecorr_post", "spectral_moments", "correlation_expansion_moments", "kubo_fft", "kubo_post",
+    "timecorr_post", "spectral_moments", "spectral_moments_nonorth", "overlap_solve", "correlation_expansion_moments", "kubo_fft", "kubo_post",
     "jackson_1d", "jackson_2d", "dos_fftgrid", "greenwood_fftgrid", "bastin_fftgrid", "LinqtError",
     "LQ_FILL_HERMITIAN_UPPER", "LQ_FILL_GENERAL", "LQ_KUBO_GREENWOOD", "LQ_KUBO_BASTIN", "LQ_ALGO_DIRECT",
     "LQ_ALGO_DOUBLING", "LQ_PREC_FP64", "LQ_PREC_INT8X4", "LQ_PREC_INT8X5", "LQ_PREC_INT8X6", "LQ_PREC_INT8X7",
@@ -273,6 +273,43 @@ def spectral_moments(H: SparseMatrix, OP, phi: np.ndarray, M: int, algo: int = L
     mu = np.zeros(M, dtype=np.complex128) if mu is None else np.array(mu, dtype=np.complex128)
     _lib.call("lq_spectral_moments", H.h, _h(OP), pp, R, M, algo, _dp(mu.view(np.float64)))
     return mu
+
+
+def spectral_moments_nonorth(H: SparseMatrix, OP, S: SparseMatrix, phi: np.ndarray, M: int, tol: float = 1e-4,
+                             maxit: int = 2000, mu: np.ndarray | None = None, info: dict | None = None) -> np.ndarray:
+    """``chebyshev::SpectralMoments_nonOrth`` (src/chebyshev_solver.cpp:203-229):
+    the Chebyshev recursion of S^-1 H with BiCGSTAB overlap solves.  S should
+    be built with ``fill=LQ_FILL_GENERAL`` from the overlap file's arrays, the
+    matrix the reference solves with.  ``info`` (a dict) receives the total
+    BiCGSTAB iterations."""
+    if OP is not None and OP.isIdentity():
+        OP = None
+    pp, R, keep = _states(phi)
+    mu = np.zeros(M, dtype=np.complex128) if mu is None else np.array(mu, dtype=np.complex128)
+    it = C.c_longlong(0)
+    _lib.call("lq_spectral_moments_nonorth", H.h, _h(OP), S.h, pp, R, M, tol, maxit, _dp(mu.view(np.float64)),
+              C.byref(it))
+    if info is not None:
+        info["iterations"] = it.value
+    return mu
+
+
+def overlap_solve(S: SparseMatrix, b: np.ndarray, tol: float = 1e-4, maxit: int = 2000):
+    """``linalg::orthogonalize`` (src/mkl_linear_algebra.cpp:100-131): x = S^-1 b
+    by BiCGSTAB; returns (x, iterations, |r| / |b|)."""
+    ctx = S.ctx
+    b = np.ascontiguousarray(b, dtype=np.complex128)
+    pb, px = ctx.alloc(b.nbytes), ctx.alloc(b.nbytes)
+    try:
+        ctx.upload(pb, b)
+        it, err = C.c_int(0), C.c_double(0.0)
+        _lib.call("lq_overlap_solve", S.h, pb, px, tol, maxit, C.byref(it), C.byref(err))
+        x = np.zeros_like(b)
+        ctx.download(px, x)
+    finally:
+        ctx.free(pb)
+        ctx.free(px)
+    return x, it.value, err.value
 
 
 def time_correlations(H: SparseMatrix, OPL, OPR, phi, M: int, nT: int, dt: float, W: float, Ec: float,

@@ -102,6 +102,9 @@ _SIGS = {
     "lq_batch_poll": (C.c_int, [_vp, C.c_int, _ip]),
     "lq_batch_result": (C.c_int, [_vp, C.c_int, _dp]),
     "lq_batch_unroll": (C.c_int, [_vp, C.c_int, _ip]),
+    "lq_overlap_solve": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int, _ip, _dp]),
+    "lq_spectral_moments_nonorth": (C.c_int, [_vp, _vp, _vp, _dp, C.c_int, C.c_int, C.c_double, C.c_int, _dp,
+                                              C.POINTER(C.c_longlong)]),
     "lq_csr_export": (C.c_int, [_vp, C.POINTER(C.c_longlong), _ip, _ip, _dp]),
     "lq_csr_from_hoppings": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, _dp, _ip, C.POINTER(_vp)]),
     "lq_csr_file_read": (C.c_int, [C.c_char_p, _ip, C.POINTER(C.c_longlong), _ip, _ip, _dp]),

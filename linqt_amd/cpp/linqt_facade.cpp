@@ -16,6 +16,15 @@ void check(int rc, const char *what) {
   }
 }
 
+// Moments file constructors: native reader (text or binary), as `>>` reads the text.
+lq_mom_header read_moments(const std::string &path, std::vector<double> &mu) {
+  lq_mom_header h{};
+  check(lq_mom_file_read(path.c_str(), &h, nullptr), "lq_mom_file_read");
+  mu.assign(2 * (size_t)h.M0 * (size_t)h.M1, 0.0);
+  check(lq_mom_file_read(path.c_str(), &h, mu.data()), "lq_mom_file_read");
+  return h;
+}
+
 lq_ctx default_context() {
   static lq_ctx ctx = nullptr;
   static std::once_flag once;
@@ -36,23 +45,20 @@ typedef std::complex<double> cplx;
 bool Sparse::OPERATOR_FromCSRFile(const std::string input, int &dim, std::vector<int> &columns,
                                   std::vector<int> &rowIndex, std::vector<cplx> &values) {
   std::cout << "\nReading the CSR file located at: " << input << std::endl;
-  std::ifstream f(input.c_str());
-  if (!f.is_open()) {
-    std::cerr << "ERROR: Can not open the matrix file";
+  // native multi-threaded parse (text or binary CSR), values bitwise as `>>` reads them
+  long long nnz = 0;
+  if (lq_csr_file_read(input.c_str(), &dim, &nnz, nullptr, nullptr, nullptr) != LQ_OK) {
+    std::cerr << "ERROR: Can not open the matrix file (" << lq_last_error() << ")";
     return false;
   }
-  int nnz = 0;
-  f >> dim >> nnz;
-  values.assign(nnz, cplx(0));
-  columns.assign(nnz, 0);
-  rowIndex.assign(dim + 1, 0);
-  for (int i = 0; i < nnz; i++) {
-    double re, im;
-    f >> re >> im;
-    values[i] = cplx(re, im);
+  values.assign((size_t)nnz, cplx(0));
+  columns.assign((size_t)nnz, 0);
+  rowIndex.assign((size_t)dim + 1, 0);
+  if (lq_csr_file_read(input.c_str(), &dim, &nnz, rowIndex.data(), columns.data(),
+                       reinterpret_cast<double *>(values.data())) != LQ_OK) {
+    std::cerr << "ERROR: " << lq_last_error() << std::endl;
+    return false;
   }
-  for (int i = 0; i < nnz; i++) f >> columns[i];
-  for (int i = 0; i < dim + 1; i++) f >> rowIndex[i];
   std::cout << "FINISH READING.Status: SUCCEED" << std::endl;
   return true;
 }
@@ -62,6 +68,40 @@ SparseMatrixType::~SparseMatrixType() {
   if (dx_) lq_free(default_context(), dx_);
   if (dy_) lq_free(default_context(), dy_);
   if (h_) lq_csr_destroy(h_);
+  if (gh_) lq_csr_destroy(gh_);
+}
+
+lq_csr SparseMatrixType::general_handle() {
+  if (!gh_)
+    check(lq_csr_create(default_context(), numRows(), (int)cols_.size(), rows_.data(), cols_.data(),
+                        reinterpret_cast<const double *>(vals_.data()), LQ_FILL_GENERAL, &gh_),
+          "lq_csr_create (general)");
+  return gh_;
+}
+
+// ---- linalg::orthogonalize (src/mkl_linear_algebra.cpp:100-131) ----------------------
+void linalg::orthogonalize(SparseMatrixType &S, std::vector<cplx> &orthogonalized, std::vector<cplx> &original) {
+  const int DIM = S.numRows();
+  lq_ctx ctx = default_context();
+  void *db = nullptr, *dx = nullptr;
+  check(lq_malloc(ctx, sizeof(cplx) * (size_t)DIM, &db), "lq_malloc");
+  check(lq_malloc(ctx, sizeof(cplx) * (size_t)DIM, &dx), "lq_malloc");
+  check(lq_memcpy_h2d(ctx, db, original.data(), sizeof(cplx) * (size_t)DIM), "lq_memcpy_h2d");
+  int its = 0;
+  double err = 0;
+  check(lq_overlap_solve(S.general_handle(), (const double *)db, (double *)dx, 1e-4, 2000, &its, &err),
+        "lq_overlap_solve");
+  orthogonalized.resize((size_t)DIM);
+  check(lq_memcpy_d2h(ctx, orthogonalized.data(), dx, sizeof(cplx) * (size_t)DIM), "lq_memcpy_d2h");
+  lq_free(ctx, db);
+  lq_free(ctx, dx);
+  double nrm = 0;
+  for (const auto &z : original) nrm += std::norm(z);
+  std::cout << "#iterations:     " << its << std::endl;
+  std::cout << "  max#iterations:" << 2000 << std::endl;
+  std::cout << "estimated error: " << err << std::endl;
+  std::cout << "  tolerance :    " << 1e-4 << std::endl;
+  std::cout << "Vector norm :    " << std::sqrt(nrm) << std::endl << std::endl;
 }
 
 void SparseMatrixType::ConvertFromCSR(std::vector<int> &rowIndex, std::vector<int> &cols, std::vector<cplx> &vals) {
@@ -337,23 +377,14 @@ Moments1D::Moments1D(std::string momfilename) {
     std::abort();
   }
   SystemLabel(momfilename.substr(0, ext));
-  std::ifstream f(momfilename.c_str());
-  assert(f.is_open());
-  int ib;
-  double db;
-  f >> ib;
-  SystemSize(ib);
-  f >> db;
-  BandWidth(db);
-  f >> db;
-  BandCenter(db);
-  f >> _numMoms;
+  std::vector<double> mu;
+  const lq_mom_header h = linqt_amd::read_moments(momfilename, mu);
+  SystemSize((int)h.n);
+  BandWidth(h.W);
+  BandCenter(h.Ec);
+  _numMoms = (size_t)h.M0;
   MomentVector(vector_t(_numMoms, 0.0));
-  for (size_t m = 0; m < _numMoms; m++) {
-    double re, im;
-    f >> re >> im;
-    (*this)(m) = value_t(re, im);
-  }
+  for (size_t m = 0; m < _numMoms; m++) (*this)(m) = value_t(mu[2 * m], mu[2 * m + 1]);
 }
 
 void Moments1D::MomentNumber(const size_t numMoms) {
@@ -403,23 +434,18 @@ Moments2D::Moments2D(std::string momfilename) {
     std::abort();
   }
   SystemLabel(momfilename.substr(0, ext));
-  std::ifstream f(momfilename.c_str());
-  assert(f.is_open());
-  int ib;
-  double db;
-  f >> ib;
-  SystemSize(ib);
-  f >> db;
-  BandWidth(db);
-  f >> db;
-  BandCenter(db);
-  f >> numMoms[0] >> numMoms[1];
+  std::vector<double> mu;
+  const lq_mom_header h = linqt_amd::read_moments(momfilename, mu);
+  SystemSize((int)h.n);
+  BandWidth(h.W);
+  BandCenter(h.Ec);
+  numMoms[0] = h.M0;
+  numMoms[1] = h.M1;
   MomentVector(vector_t((size_t)numMoms[0] * numMoms[1], 0.0));
   for (int m0 = 0; m0 < numMoms[0]; m0++)
     for (int m1 = 0; m1 < numMoms[1]; m1++) {
-      double re, im;
-      f >> re >> im;
-      (*this)(m0, m1) = value_t(re, im);
+      const size_t q = (size_t)m0 * numMoms[1] + m1;
+      (*this)(m0, m1) = value_t(mu[2 * q], mu[2 * q + 1]);
     }
 }
 
@@ -473,27 +499,20 @@ MomentsTD::MomentsTD(std::string momfilename) : _numMoms(1), _maxTimeStep(1), _t
     std::abort();
   }
   SystemLabel(momfilename.substr(0, ext));
-  std::ifstream f(momfilename.c_str());
-  assert(f.is_open());
-  int ib;
-  double db;
-  f >> ib;
-  SystemSize(ib);
-  f >> db;
-  BandWidth(db);
-  f >> db;
-  BandCenter(db);
-  f >> ib;
-  MaxTimeStep(ib);
-  f >> db;
-  TimeDiff(db);
-  f >> _numMoms >> _maxTimeStep;
+  std::vector<double> mu;
+  const lq_mom_header h = linqt_amd::read_moments(momfilename, mu);
+  SystemSize((int)h.n);
+  BandWidth(h.W);
+  BandCenter(h.Ec);
+  MaxTimeStep(h.M1);
+  TimeDiff(h.dt);
+  _numMoms = (size_t)h.M0;
+  _maxTimeStep = (size_t)h.M1;
   MomentVector(vector_t(_numMoms * _maxTimeStep, 0.0));
   for (size_t m = 0; m < _numMoms; m++)
     for (size_t n = 0; n < _maxTimeStep; n++) {
-      double re, im;
-      f >> re >> im;
-      (*this)(m, n) = value_t(re, im);
+      const size_t q = m * _maxTimeStep + n;
+      (*this)(m, n) = value_t(mu[2 * q], mu[2 * q + 1]);
     }
 }
 
@@ -679,6 +698,59 @@ int TimeDependentCorrelations(SparseMatrixType &OPL, SparseMatrixType &OPR, Mome
                              chebMoms.BandWidth(), chebMoms.BandCenter(),
                              reinterpret_cast<double *>(chebMoms.MomentVector().data())),
         "lq_time_correlations");
+  return 0;
+}
+
+// ---- Moments1D_nonOrth (src/chebyshev_moments.cpp:63-153): host-vector semantics ----
+void Moments1D_nonOrth::SetInitVectors_nonOrthogonal(vector_t &T0) {
+  const size_t dim = SystemSize();
+  assert(T0.size() == dim && S_);
+  Chebyshev0().assign(dim, value_t(0));
+  Chebyshev1().assign(dim, value_t(0));
+  vector_t tmp(dim, 0.0);
+  linalg::orthogonalize(*S_, Chebyshev0(), T0);
+  Hamiltonian().Multiply(Chebyshev0(), tmp);
+  linalg::orthogonalize(*S_, Chebyshev1(), tmp);
+}
+
+void Moments1D_nonOrth::SetInitVectors_nonOrthogonal(SparseMatrixType &OP, vector_t &T0) {
+  const size_t dim = SystemSize();
+  assert(T0.size() == dim && S_);
+  Chebyshev0().assign(dim, value_t(0));
+  Chebyshev1().assign(dim, value_t(0));
+  vector_t tmp(dim, 0.0);
+  // T0 = OP S^-1 phi, T1 = S^-1 H T0 (the reference first forms OP phi and
+  // H OP phi, which the later assignments overwrite)
+  linalg::orthogonalize(*S_, tmp, T0);
+  Chebyshev1() = tmp;
+  OP.Multiply(Chebyshev1(), Chebyshev0());
+  Hamiltonian().Multiply(Chebyshev0(), tmp);
+  linalg::orthogonalize(*S_, Chebyshev1(), tmp);
+}
+
+int Moments1D_nonOrth::Iterate_nonOrthogonal() {
+  const size_t dim = Chebyshev0().size();
+  vector_t tmp(dim, 0.0), tmp2(dim, 0.0);
+  Hamiltonian().Multiply(Chebyshev1(), tmp);
+  linalg::orthogonalize(*S_, tmp2, tmp);
+  for (size_t i = 0; i < dim; i++) Chebyshev0()[i] = 2.0 * tmp2[i] - Chebyshev0()[i];
+  Chebyshev0().swap(Chebyshev1());
+  return 0;
+}
+
+// SpectralMoments_nonOrth (src/chebyshev_solver.cpp:203-229): one device solve.
+int SpectralMoments_nonOrth(SparseMatrixType &OP, Moments1D_nonOrth &chebMoms, qstates::generator &gen) {
+  int R = 0;
+  std::vector<cplx> phi = collect_states(gen, (int)chebMoms.SystemSize(), &R);
+  if (R == 0) return 0;
+  assert(chebMoms.S());
+  long long its = 0;
+  check(lq_spectral_moments_nonorth(chebMoms.Hamiltonian().handle(), OP.isIdentity() ? nullptr : OP.handle(),
+                                    chebMoms.S()->general_handle(), reinterpret_cast<const double *>(phi.data()), R,
+                                    (int)chebMoms.HighestMomentNumber(), 1e-4, 2000,
+                                    reinterpret_cast<double *>(chebMoms.MomentVector().data()), &its),
+        "lq_spectral_moments_nonorth");
+  std::cout << "BiCGSTAB overlap solves: " << its << " iterations in total (tolerance 1e-4)" << std::endl;
   return 0;
 }
 

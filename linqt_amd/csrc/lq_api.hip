@@ -894,6 +894,69 @@ int lq_csr_create(lq_ctx ctx, int n, int nnz, const int *rowptr, const int *col,
     set_error("lq_csr_create: rowptr[0]=%d rowptr[n]=%d, expected 0 and nnz=%d", rowptr[0], rowptr[n], nnz);
     return LQ_ERR_ARG;
   }
+  // Rows with non-decreasing columns (every file the reference's writers
+  // produce): the completed CSR is built by counting, in O(nnz).  A row i's
+  // completed entries are the mirrors of the stored (j, i), j < i, in
+  // increasing j, then its own entries with col >= i in input order -- the
+  // order the stable sort below produces, ties included.
+  bool sorted = true;
+  for (int i = 0; i < n && sorted; ++i) {
+    if (rowptr[i + 1] < rowptr[i]) break;  // reported below
+    for (int q = rowptr[i] + 1; q < rowptr[i + 1]; ++q)
+      if (col[q] < col[q - 1]) {
+        sorted = false;
+        break;
+      }
+  }
+  if (sorted) {
+    std::vector<int> cnt((size_t)n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+      if (rowptr[i + 1] < rowptr[i]) {
+        set_error("lq_csr_create: rowptr decreases at row %d", i);
+        return LQ_ERR_ARG;
+      }
+      for (int q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+        const int j = col[q];
+        if (j < 0 || j >= n) {
+          set_error("lq_csr_create: column %d out of range in row %d", j, i);
+          return LQ_ERR_ARG;
+        }
+        if (fill == LQ_FILL_GENERAL) {
+          cnt[i + 1]++;
+        } else if (j >= i) {
+          cnt[i + 1]++;
+          if (j != i) cnt[j + 1]++;
+        }
+      }
+    }
+    for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    lq_csr A = new lq_csr_s();
+    A->n = n;
+    A->fill = fill;
+    A->rowptr = cnt;
+    const size_t total = (size_t)cnt[n];
+    A->col.resize(total);
+    A->val.resize(total);
+    A->kind.resize(total);
+    std::vector<int> cur(cnt.begin(), cnt.end() - 1);
+    for (int i = 0; i < n; ++i)
+      for (int q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+        const int j = col[q];
+        if (fill != LQ_FILL_GENERAL && j < i) continue;  // MKL's upper-fill descriptor ignores the lower triangle
+        const std::complex<double> v(val[2 * (size_t)q], val[2 * (size_t)q + 1]);
+        int p = cur[i]++;
+        A->col[p] = j;
+        A->val[p] = v;
+        A->kind[p] = (signed char)(j == i ? 2 : 0);
+        if (fill != LQ_FILL_GENERAL && j != i) {  // mirror: row j gains (j, i) after its earlier mirrors
+          p = cur[j]++;
+          A->col[p] = i;
+          A->val[p] = std::conj(v);
+          A->kind[p] = 1;
+        }
+      }
+    return csr_adopt(ctx, A, out);
+  }
   std::vector<Entry> e;
   e.reserve((size_t)nnz * (fill == LQ_FILL_HERMITIAN_UPPER ? 2 : 1));
   for (int i = 0; i < n; ++i) {

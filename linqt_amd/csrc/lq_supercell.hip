@@ -306,8 +306,11 @@ extern "C" int lq_csr_from_hoppings(lq_ctx ctx, int nwann, int nhop, const int *
   int *d_count = nullptr;
   LQ_TRY(dbuf.alloc(sizeof(int) * ((size_t)n + 1), (void **)&d_count));
   const unsigned blocks = (unsigned)((n + 255) / 256);
+  hipEvent_t ev_count;
+  LQ_TRY(time_begin(ctx, kCatPost, &ev_count));
   hipLaunchKernelGGL(k_sc_count, dim3(blocks), dim3(256), 0, ctx->stream, g, d_count);
   LQ_HIP_CHECK(hipGetLastError());
+  LQ_TRY(time_end(ctx, kCatPost, ev_count, 4.0 * n));
   lq_csr A = new lq_csr_s();
   A->n = n;
   A->fill = LQ_FILL_HERMITIAN_UPPER;
@@ -345,12 +348,16 @@ extern "C" int lq_csr_from_hoppings(lq_ctx ctx, int nwann, int nhop, const int *
       rc = LQ_ERR_HIP;
       break;
     }
+    hipEvent_t ev_fill;
+    if ((rc = time_begin(ctx, kCatPost, &ev_fill)) != LQ_OK) break;
     hipLaunchKernelGGL(k_sc_fill, dim3(blocks), dim3(256), 0, ctx->stream, g, d_rowptr, d_col, d_val, d_kind);
     if (hipGetLastError() != hipSuccess) {
       set_error("lq_csr_from_hoppings: fill launch failed");
       rc = LQ_ERR_HIP;
       break;
     }
+    // bytes written: column, value and kind per completed entry, plus the row pointers read
+    if ((rc = time_end(ctx, kCatPost, ev_fill, 21.0 * (double)total + 4.0 * n)) != LQ_OK) break;
     A->col.resize((size_t)total);
     A->val.resize((size_t)total);
     A->kind.resize((size_t)total);

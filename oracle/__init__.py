@@ -66,6 +66,10 @@ def lib():
             getattr(L, fn).argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp]
         L.lqo_supercell_csr.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _ip, _ip, C.POINTER(C.c_longlong), _ip,
                                         _ip, _dp]
+        L.lqo_overlap_solve.argtypes = [C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int, _dp]
+        L.lqo_spectral_moments_nonorth.argtypes = [C.c_void_p, C.c_void_p, C.c_int, _ip, _ip, _dp, _dp, C.c_int,
+                                                   C.c_int, C.c_double, C.c_int, _dp]
+        L.lqo_csr_read_stream.argtypes = [C.c_char_p, _ip, C.POINTER(C.c_longlong), _ip, _ip, _dp]
         L.lqo_omp_threads.restype = C.c_int
         L.lqo_omp_cheb_step.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.lqo_omp_momtable.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp]
@@ -287,6 +291,48 @@ def supercell_csr(nwann: int, cell, edge, val, dims):
     _check(L.lqo_supercell_csr(nwann, nh, _i(cell), _i(edge), _d(v.view(np.float64)), _i(d), C.byref(n),
                                C.byref(nnz), _i(rowptr), _i(col), _d(out.view(np.float64))), "supercell_csr")
     return n.value, rowptr, col[: nnz.value], out[: nnz.value]
+
+
+def overlap_solve(S, b, tol=1e-4, maxit=2000):
+    """linalg::orthogonalize: x = S^-1 b by Eigen's BiCGSTAB on the stored CSR
+    (io.CSR) as a general matrix.  Returns (x, iterations, error)."""
+    b = np.ascontiguousarray(b, dtype=np.complex128)
+    x = np.zeros_like(b)
+    err = C.c_double(0)
+    rp = np.ascontiguousarray(S.rowptr, np.int32)
+    cl = np.ascontiguousarray(S.col, np.int32)
+    v = np.ascontiguousarray(S.val, np.complex128)
+    it = lib().lqo_overlap_solve(S.n, _i(rp), _i(cl), _d(v.view(np.float64)), _d(b.view(np.float64)),
+                                 _d(x.view(np.float64)), tol, maxit, C.byref(err))
+    _check(it, "overlap_solve")
+    return x, it, err.value
+
+
+def spectral_moments_nonorth(H: "Op", OP, S, phi: np.ndarray, M: int, tol=1e-4, maxit=2000, mu=None):
+    """SpectralMoments_nonOrth; S is an io.CSR used as a general matrix."""
+    phi = np.ascontiguousarray(np.atleast_2d(phi), dtype=np.complex128)
+    R, n = phi.shape
+    mu = np.zeros(M, dtype=np.complex128) if mu is None else mu
+    rp = np.ascontiguousarray(S.rowptr, np.int32)
+    cl = np.ascontiguousarray(S.col, np.int32)
+    v = np.ascontiguousarray(S.val, np.complex128)
+    _check(lib().lqo_spectral_moments_nonorth(H.h, _h(OP), n, _i(rp), _i(cl), _d(v.view(np.float64)),
+                                              _d(phi.view(np.float64)), R, M, tol, maxit,
+                                              _d(mu.view(np.float64))), "spectral_moments_nonorth")
+    return mu
+
+
+def csr_read_stream(path: str):
+    """The reference's stream CSR reader; returns (n, rowptr, col, val)."""
+    L = lib()
+    n, nnz = C.c_int(0), C.c_longlong(0)
+    _check(L.lqo_csr_read_stream(str(path).encode(), C.byref(n), C.byref(nnz), None, None, None), "csr_read_stream")
+    rowptr = np.zeros(n.value + 1, np.int32)
+    col = np.zeros(max(nnz.value, 1), np.int32)
+    val = np.zeros(max(nnz.value, 1), np.complex128)
+    _check(L.lqo_csr_read_stream(str(path).encode(), C.byref(n), C.byref(nnz), _i(rowptr), _i(col),
+                                 _d(val.view(np.float64))), "csr_read_stream")
+    return n.value, rowptr, col[: nnz.value], val[: nnz.value]
 
 
 def omp_threads() -> int:

@@ -1298,3 +1298,150 @@ int lqo_supercell_csr(int nwann, int nhop, const int *cell, const int *edge, con
   return 0;
 }
 
+/* Sparse::OPERATOR_FromCSRFile (src/sparse_matrix.cpp:3-50): sequential
+ * token-by-token extraction from a buffered stream. */
+int lqo_csr_read_stream(const char *path, int *n, long long *nnz, int *rowptr, int *col, double *val) {
+  FILE *f = fopen(path, "r");
+  if (!f) return -1;
+  int dim = 0, nz = 0;
+  if (fscanf(f, "%d %d", &dim, &nz) != 2 || dim <= 0 || nz < 0) {
+    fclose(f);
+    return -1;
+  }
+  *n = dim;
+  *nnz = nz;
+  if (!rowptr) {
+    fclose(f);
+    return 0;
+  }
+  int bad = 0;
+  for (int i = 0; i < nz && !bad; ++i) bad = fscanf(f, "%lf %lf", &val[2 * i], &val[2 * i + 1]) != 2;
+  for (int i = 0; i < nz && !bad; ++i) bad = fscanf(f, "%d", &col[i]) != 1;
+  for (int i = 0; i <= dim && !bad; ++i) bad = fscanf(f, "%d", &rowptr[i]) != 1;
+  fclose(f);
+  return bad ? -1 : 0;
+}
+
+/* ======================================================================== */
+/* Non-orthogonal basis: Eigen::BiCGSTAB (Eigen/src/IterativeLinearSolvers/ */
+/* BiCGSTAB.h, 3.3/3.4) with DiagonalPreconditioner, as                     */
+/* linalg::orthogonalize (src/mkl_linear_algebra.cpp:100-131) runs it.      */
+/* ======================================================================== */
+
+/* y = S x, S general CSR, row sums in stored order */
+static void gen_mv(int n, const int *rp, const int *cl, const cplx *v, const cplx *x, cplx *y) {
+  for (int i = 0; i < n; ++i) {
+    cplx t = 0;
+    for (int q = rp[i]; q < rp[i + 1]; ++q) t += v[q] * x[cl[q]];
+    y[i] = t;
+  }
+}
+
+static double sqnorm(int n, const cplx *x) {
+  double s = 0;
+  for (int i = 0; i < n; ++i) s += creal(x[i]) * creal(x[i]) + cimag(x[i]) * cimag(x[i]);
+  return s;
+}
+
+int lqo_overlap_solve(int n, const int *rp, const int *cl, const double *val_, const double *b_, double *x_,
+                      double tol, int maxit, double *error) {
+  if (n <= 0 || !rp || !b_ || !x_) return -1;
+  const cplx *v = (const cplx *)val_, *b = (const cplx *)b_;
+  cplx *x = (cplx *)x_;
+  /* DiagonalPreconditioner::factorize: 1/diag where a nonzero diagonal is stored, else 1 */
+  cplx *dinv = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    dinv[i] = 1.0;
+    for (int q = rp[i]; q < rp[i + 1]; ++q)
+      if (cl[q] == i) {
+        if (v[q] != 0) dinv[i] = 1.0 / v[q];
+        break;
+      }
+  }
+  cplx *r = (cplx *)calloc((size_t)n, sizeof(cplx)), *r0 = (cplx *)calloc((size_t)n, sizeof(cplx));
+  cplx *p = (cplx *)calloc((size_t)n, sizeof(cplx)), *vv = (cplx *)calloc((size_t)n, sizeof(cplx));
+  cplx *y = (cplx *)calloc((size_t)n, sizeof(cplx)), *z = (cplx *)calloc((size_t)n, sizeof(cplx));
+  cplx *s = (cplx *)calloc((size_t)n, sizeof(cplx)), *t = (cplx *)calloc((size_t)n, sizeof(cplx));
+  for (int i = 0; i < n; ++i) x[i] = 0; /* solve() starts from zero */
+  const double rhs_sq = sqnorm(n, b);
+  int it = 0;
+  if (rhs_sq == 0) {
+    if (error) *error = 0;
+  } else {
+    for (int i = 0; i < n; ++i) r[i] = r0[i] = b[i]; /* r = b - S*0 */
+    double r0_sq = sqnorm(n, r0);
+    cplx rho = 1, alpha = 1, w = 1;
+    const double tol2 = tol * tol * rhs_sq;
+    const double eps = 2.220446049250313e-16, eps2 = eps * eps;
+    int restarts = 0;
+    while (sqnorm(n, r) > tol2 && it < maxit) {
+      const cplx rho_old = rho;
+      rho = vdot(n, r0, r);
+      if (cabs(rho) < eps2 * r0_sq) { /* r became too orthogonal to r0: restart */
+        gen_mv(n, rp, cl, v, x, t);
+        for (int i = 0; i < n; ++i) r[i] = r0[i] = b[i] - t[i];
+        rho = r0_sq = sqnorm(n, r);
+        if (restarts++ == 0) it = 0;
+      }
+      const cplx beta = (rho / rho_old) * (alpha / w);
+      for (int i = 0; i < n; ++i) {
+        p[i] = r[i] + beta * (p[i] - w * vv[i]);
+        y[i] = dinv[i] * p[i];
+      }
+      gen_mv(n, rp, cl, v, y, vv);
+      alpha = rho / vdot(n, r0, vv);
+      for (int i = 0; i < n; ++i) {
+        s[i] = r[i] - alpha * vv[i];
+        z[i] = dinv[i] * s[i];
+      }
+      gen_mv(n, rp, cl, v, z, t);
+      const double tmp = sqnorm(n, t);
+      w = tmp > 0 ? vdot(n, t, s) / tmp : 0;
+      for (int i = 0; i < n; ++i) {
+        x[i] += alpha * y[i] + w * z[i];
+        r[i] = s[i] - w * t[i];
+      }
+      ++it;
+    }
+    if (error) *error = sqrt(sqnorm(n, r) / rhs_sq);
+  }
+  free(dinv); free(r); free(r0); free(p); free(vv); free(y); free(z); free(s); free(t);
+  return it;
+}
+
+int lqo_spectral_moments_nonorth(const lqo_op *H, const lqo_op *OP, int n, const int *srp, const int *scl,
+                                 const double *sval, const double *phi, int R, int M, double tol, int maxit,
+                                 double *mu_) {
+  if (!H || H->n != n || (OP && OP->n != n) || !phi || R <= 0 || M <= 0 || !mu_) return -1;
+  cplx *mu = (cplx *)mu_;
+  cplx *c0 = (cplx *)malloc(sizeof(cplx) * (size_t)n), *c1 = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+  cplx *tmp = (cplx *)malloc(sizeof(cplx) * (size_t)n), *tmp2 = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+  for (int r = 0; r < R; ++r) {
+    const cplx *Phi = (const cplx *)phi + (size_t)r * n;
+    /* SetInitVectors_nonOrthogonal: T0 = [OP] S^-1 phi, T1 = S^-1 H T0 */
+    if (!OP) {
+      lqo_overlap_solve(n, srp, scl, sval, (const double *)Phi, (double *)c0, tol, maxit, NULL);
+    } else {
+      lqo_overlap_solve(n, srp, scl, sval, (const double *)Phi, (double *)tmp, tol, maxit, NULL);
+      op_mv_c(OP, 1.0, tmp, 0.0, c0);
+    }
+    op_mv_c(H, 1.0, c0, 0.0, tmp);
+    lqo_overlap_solve(n, srp, scl, sval, (const double *)tmp, (double *)c1, tol, maxit, NULL);
+    for (int m = 0; m < M; ++m) {
+      double scal = 2.0 / R;
+      if (m == 0) scal *= 0.5;
+      mu[m] += scal * vdot(n, Phi, c0);
+      if (m + 1 == M) break; /* the reference's last Iterate leaves mu unchanged */
+      /* Iterate_nonOrthogonal: T0 <- 2 S^-1 H T1 - T0, swap */
+      op_mv_c(H, 1.0, c1, 0.0, tmp);
+      lqo_overlap_solve(n, srp, scl, sval, (const double *)tmp, (double *)tmp2, tol, maxit, NULL);
+      for (int i = 0; i < n; ++i) c0[i] = 2.0 * tmp2[i] - c0[i];
+      cplx *sw = c0;
+      c0 = c1;
+      c1 = sw;
+    }
+  }
+  free(c0); free(c1); free(tmp); free(tmp2);
+  return 0;
+}
+

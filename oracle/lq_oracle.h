@@ -14,6 +14,8 @@
  *                                         include/Kubo_solver_FFT.hpp:99-107
  *   - Kubo_solver_FFT (Greenwood/Bastin). src/Kubo_solver_FFT.cpp, src/Greenwood_FFTs.cpp, src/Bastin_FFTs.cpp,
  *                                         src/Kubo_solver_FFT_postProcess.cpp
+ *   - non-orthogonal SpectralMoments .... src/chebyshev_solver.cpp:203-229, src/chebyshev_moments.cpp:63-153,
+ *                                         src/mkl_linear_algebra.cpp:100-131 (Eigen BiCGSTAB)
  *   - supercell operators ............... utilities/wannier2sparse/src/hopping_list.cpp:156-285
  *   - FFT-grid post-processors .......... src/spectralFunctionFromChebmom_FFTgrid.cpp:35-73,
  *                                         src/kuboGreenwoodFromChebmom_FFTgrid.cpp:35-76,
@@ -126,6 +128,24 @@ int lqo_linear_kernel(int kind, const double *mu, int M0, int M1, int n, double 
 int lqo_linear_grid(int kind, const double *mu, int M0, int M1, int n, double W, double Ec,
                     int num_div, double *E_out, double *out);
 
+/* ---- non-orthogonal basis ------------------------------------------------ */
+/* linalg::orthogonalize (src/mkl_linear_algebra.cpp:100-131): x = S^-1 b by
+ * Eigen's BiCGSTAB with its diagonal preconditioner, zero initial guess,
+ * stopping when |r|^2 <= tol^2 |b|^2 or after maxit iterations.  S is the CSR
+ * (n, rowptr, col, val) read as a general matrix -- the reference maps the
+ * stored arrays into an Eigen::SparseMatrix, so a Hermitian-upper file acts
+ * by its upper triangle.  Returns the iteration count (>= 0). */
+int lqo_overlap_solve(int n, const int *rowptr, const int *col, const double *val, const double *b,
+                      double *x, double tol, int maxit, double *error);
+/* chebyshev::SpectralMoments_nonOrth (src/chebyshev_solver.cpp:203-229) with
+ * Moments1D_nonOrth::SetInitVectors_nonOrthogonal / Iterate_nonOrthogonal
+ * (src/chebyshev_moments.cpp:63-153): T0 = [OP] S^-1 phi, T1 = S^-1 H T0,
+ * T_{m+1} = 2 S^-1 H T_m - T_{m-1}; mu_m += (2/R)(1/2 if m = 0) <phi|T_m>.
+ * S as in lqo_overlap_solve; OP == NULL means the identity.  mu accumulated. */
+int lqo_spectral_moments_nonorth(const lqo_op *H, const lqo_op *OP, int n, const int *s_rowptr, const int *s_col,
+                                 const double *s_val, const double *phi, int R, int M, double tol, int maxit,
+                                 double *mu);
+
 /* ---- supercell operators (wannier2sparse) ------------------------------ */
 /* hopping_list::wrap_in_supercell + save_hopping_list_as_csr
  * (utilities/wannier2sparse/src/hopping_list.cpp:156-225, 227-285) on a
@@ -140,6 +160,11 @@ int lqo_linear_grid(int kind, const double *mu, int M0, int M1, int n, double W,
  * (every hopping appears Ncells times in the supercell). */
 int lqo_supercell_csr(int nwann, int nhop, const int *cell, const int *edge, const double *val,
                       const int *dims, int *n, long long *nnz, int *rowptr, int *col, double *out_val);
+
+/* Sparse::OPERATOR_FromCSRFile (src/sparse_matrix.cpp:3-50): the text CSR
+ * read one token at a time from a stream (here C stdio, as the reference
+ * reads with ifstream >>).  rowptr == NULL: *n and *nnz only. */
+int lqo_csr_read_stream(const char *path, int *n, long long *nnz, int *rowptr, int *col, double *val);
 
 /* ---- OpenMP CPU baseline (same algorithm, threads over rows/pairs) ----- */
 int  lqo_omp_threads(void);

@@ -190,3 +190,31 @@ def test_time_correlation_drivers_match_oracle(chain_dir, oracle):
     t, v = _two_col(rows[0])
     to, vo = oracle.timecorr_post(oracle.jackson_td(mu, W, 100.0), n, W, Ec, dt, 0.3)
     assert np.allclose(t, to, rtol=1e-5) and rel_inf(v, vo) < 1e-5
+
+
+@pytest.mark.gpu
+def test_nonorth_driver_matches_oracle(tmp_path, oracle):
+    """inline_compute-kpm-spectralOp-nonOrth: operators/LABEL.{HAM,S}.CSR and
+    BOUNDS in, SpectralOp-nonOrth-... / Spectral-nonOrth-OP1... out; the
+    overlap solves run at the reference's tolerance (1e-4), so the moments
+    match the oracle's tight-tolerance ones to that accuracy."""
+    os.makedirs(tmp_path / "operators")
+    m = lattices.square_anderson(16, W=1.0, seed=3)
+    n = m.n
+    io.write_csr(str(tmp_path / "operators" / "sq.HAM.CSR"), m)
+    rows = np.repeat(np.arange(n), np.diff(m.rowptr))
+    sval = np.where(m.col == rows, 1.0 + 0j, 0.03 * np.sign(m.val.real) + 0j)
+    S = io.CSR(n, m.rowptr, m.col, sval)
+    io.write_csr(str(tmp_path / "operators" / "sq.S.CSR"), S)
+    (tmp_path / "BOUNDS").write_text("-4.6 4.6\n")
+    _run("inline_compute-kpm-spectralOp-nonOrth", ["sq", "1", 40], tmp_path)
+    out = tmp_path / "Spectral-nonOrth-OP1sqKPM_M40_statedefault.chebmom1D-nonOrth"
+    assert out.exists(), sorted(os.listdir(tmp_path))
+    t = np.array(out.read_text().split(), dtype=float)
+    assert int(t[0]) == n and int(t[3]) == 40
+    mu = t[4:4 + 80].reshape(40, 2) @ np.array([1, 1j])
+    H = oracle.Op.from_csr(io.read_csr(str(tmp_path / "operators" / "sq.HAM.CSR"))).rescale(1 / 4.6, 0.0)
+    phi = io.default_state(n)
+    ref = oracle.spectral_moments_nonorth(H, None, io.read_csr(str(tmp_path / "operators" / "sq.S.CSR")),
+                                          phi, 40, tol=1e-14)
+    assert np.abs(mu - ref).max() <= 1e-3 * np.abs(ref).max()

@@ -332,3 +332,60 @@ def test_time_correlations_vs_oracle(ctx, oracle):
     t, v = lq.timecorr_post(ctx, tdj, H.n, W, Ec, dt, 0.7)
     to, vo = oracle.timecorr_post(tdo, H.n, W, Ec, dt, 0.7)
     assert np.allclose(t, to) and rel_inf(v, vo) < 1e-9
+
+
+@pytest.mark.parametrize("fill", [lq.LQ_FILL_HERMITIAN_UPPER, lq.LQ_FILL_GENERAL])
+def test_csr_create_counting_path_equals_sorting_path(ctx, fill):
+    """Rows with sorted columns take the O(nnz) counting construction; the same
+    entries with each row reversed (duplicates and lower-triangle entries
+    included) take the stable sort.  Both give the same operator bitwise."""
+    say("csr create paths", fill)
+    rng = np.random.default_rng(12)
+    n = 700
+    rows, cols = [], []
+    for i in range(n):
+        js = sorted(rng.choice(n, size=6, replace=False).tolist() + [i, i])  # a duplicated diagonal
+        if i % 7 == 0:
+            js.append(js[-1])  # a duplicated off-diagonal entry
+        rows += [i] * len(js)
+        cols += js
+    rows, cols = np.array(rows), np.array(cols, np.int32)
+    vals = rng.normal(size=cols.size) + 1j * rng.normal(size=cols.size)
+    rowptr = np.zeros(n + 1, np.int32)
+    np.add.at(rowptr, rows + 1, 1)
+    rowptr = np.cumsum(rowptr).astype(np.int32)
+    A = lq.SparseMatrix(ctx, n, rowptr, cols, vals, fill=fill)
+    rc, rv = cols.copy(), vals.copy()
+    for i in range(n):
+        s = slice(rowptr[i], rowptr[i + 1])
+        rc[s], rv[s] = rc[s][::-1], rv[s][::-1]
+    B = lq.SparseMatrix(ctx, n, rowptr, rc, rv, fill=fill)
+    ea, eb = A.export(), B.export()
+    assert A.info() == B.info() and np.array_equal(ea.rowptr, eb.rowptr)
+    # the stable sort keeps equal (row, col) entries in input order, which the reversal flips:
+    # compare per (row, col) sums, then the operators themselves
+    x = rng.normal(size=(n, 2)) + 1j * rng.normal(size=(n, 2))
+    ya, yb = A.Multiply(1.0, x), B.Multiply(1.0, x)
+    assert np.abs(ya - yb).max() <= 1e-13 * np.abs(ya).max()
+    # without duplicates the two paths agree bitwise
+    keep = np.ones(cols.size, bool)
+    for i in range(n):
+        seen = set()
+        for q in range(rowptr[i], rowptr[i + 1]):
+            if cols[q] in seen:
+                keep[q] = False
+            seen.add(cols[q])
+    r2 = rows[keep]
+    rp2 = np.zeros(n + 1, np.int32)
+    np.add.at(rp2, r2 + 1, 1)
+    rp2 = np.cumsum(rp2).astype(np.int32)
+    c2, v2 = cols[keep], vals[keep]
+    A2 = lq.SparseMatrix(ctx, n, rp2, c2, v2, fill=fill)
+    rc2, rv2 = c2.copy(), v2.copy()
+    for i in range(n):
+        s = slice(rp2[i], rp2[i + 1])
+        rc2[s], rv2[s] = rc2[s][::-1], rv2[s][::-1]
+    B2 = lq.SparseMatrix(ctx, n, rp2, rc2, rv2, fill=fill)
+    e2a, e2b = A2.export(), B2.export()
+    assert np.array_equal(e2a.col, e2b.col) and np.array_equal(e2a.val.view(np.int64), e2b.val.view(np.int64))
+    assert np.array_equal(A2.Multiply(1.0, x), B2.Multiply(1.0, x))

@@ -557,3 +557,78 @@ def test_supercell_filters(oracle):
     assert n == 4 and col.size == 0 and list(rowptr) == [0] * 5
     n, rowptr, col, v = oracle.supercell_csr(2, np.zeros((0, 3)), np.zeros((0, 2)), [], (2, 1, 1))
     assert n == 4 and col.size == 0
+
+
+# ---- non-orthogonal basis (SpectralMoments_nonOrth, BiCGSTAB overlap solves) ----
+def _overlap(rng, n, eps=0.08):
+    """A Hermitian positive overlap S = I + eps X stored as its upper triangle
+    (the reference's CSR files), and the matrix the reference solves with: the
+    stored arrays read as a general matrix, i.e. triu(S)."""
+    X = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    X = (X + X.conj().T) / 2
+    X[np.abs(X) < 1.2] = 0  # sparse
+    S = np.eye(n) + eps * X
+    U = np.triu(S)
+    rows, cols = np.nonzero(U)
+    rowptr = np.zeros(n + 1, np.int32)
+    np.add.at(rowptr, rows + 1, 1)
+    rowptr = np.cumsum(rowptr).astype(np.int32)
+    return io.CSR(n, rowptr, cols.astype(np.int32), U[rows, cols].astype(np.complex128)), U
+
+
+def test_overlap_solve_matches_dense_solve(oracle):
+    rng = np.random.default_rng(11)
+    n = 60
+    S, U = _overlap(rng, n)
+    b = rng.normal(size=n) + 1j * rng.normal(size=n)
+    x, it, err = oracle.overlap_solve(S, b, tol=1e-14, maxit=2000)
+    assert err <= 1e-14 and 0 < it < 100
+    assert np.allclose(x, np.linalg.solve(U, b), rtol=0, atol=1e-12 * np.abs(b).max())
+    x4, it4, err4 = oracle.overlap_solve(S, b)  # the reference's tolerance 1e-4
+    assert err4 <= 1e-4 and it4 <= it
+    r = b - U @ x4
+    assert np.linalg.norm(r) <= 1e-4 * np.linalg.norm(b) * (1 + 1e-12)
+    z, it0, e0 = oracle.overlap_solve(S, np.zeros(n, complex))
+    assert it0 == 0 and not z.any()
+
+
+def test_nonorth_moments_with_identity_overlap_equal_orthogonal_ones(oracle):
+    """S = I: every BiCGSTAB solve returns its right-hand side exactly
+    (alpha = 1, s = 0), so the recursion is the orthogonal one."""
+    m = lattices.square_anderson(6, W=1.0, seed=4)
+    H = oracle.Op.from_csr(m).rescale(1 / 4.6, 0.0)
+    n = m.n
+    I = io.CSR(n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.ones(n, np.complex128))
+    phi = lattices.random_phase_states(n, 2, seed=5)
+    mu = oracle.spectral_moments_nonorth(H, None, I, phi, 40)
+    ref = oracle.spectral_moments(H, None, phi, 40)
+    assert np.abs(mu - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("with_op", [False, True])
+def test_nonorth_moments_match_dense_chebyshev_of_generalised_problem(oracle, with_op):
+    """mu_m = (2/R)(1/2 if m = 0) sum_r phi_r^H T_m(A) [OP] U^-1 phi_r with
+    A = U^-1 H~ and U the stored triangle of S, computed densely (tight solver
+    tolerance)."""
+    rng = np.random.default_rng(3 + with_op)
+    n, M, R = 48, 30, 2
+    Hm = io.CSR(n, *random_upper_csr(n, 0.15, seed=21 + with_op))
+    Dh = dense_herm(n, Hm.rowptr, Hm.col, Hm.val)
+    a = 0.9 / np.abs(np.linalg.eigvalsh(Dh)).max()
+    H = oracle.Op.from_csr(Hm).rescale(a, 0.0)
+    S, U = _overlap(rng, n, eps=0.05)
+    OPm = io.CSR(n, *random_upper_csr(n, 0.1, seed=31)) if with_op else None
+    OP = oracle.Op.from_csr(OPm) if with_op else None
+    phi = lattices.random_phase_states(n, R, seed=9)
+    mu = oracle.spectral_moments_nonorth(H, OP, S, phi, M, tol=1e-15)
+    A = np.linalg.solve(U, a * Dh)
+    ref = np.zeros(M, complex)
+    for r in range(R):
+        t0 = np.linalg.solve(U, phi[r])
+        if with_op:
+            t0 = dense_herm(n, OPm.rowptr, OPm.col, OPm.val) @ t0
+        t1 = A @ t0
+        for m in range(M):
+            ref[m] += (2.0 / R) * (0.5 if m == 0 else 1.0) * np.vdot(phi[r], t0)
+            t0, t1 = t1, 2 * A @ t1 - t0
+    assert np.abs(mu - ref).max() <= 1e-10 * np.abs(ref).max()

@@ -6,14 +6,14 @@
 // table is one complex GEMM  C = A^H B  on FP64 MFMA (v_mfma_f64_16x16x4_f64):
 //   Re C = Ar Br + Ai Bi,   Im C = Ar Bi - Ai Br.
 //
-// Two kernels: k_contract_f64_lds (128 x 128 block tile staged through LDS,
-// used for tables of at least 128 x 128) and k_contract_f64 (64 x 64 tile,
-// operands straight from global memory, for small tables).  The table is a
-// GEMM with a tiny output and a huge K, so it is bound by how often A and B
-// are re-read: the LDS kernel reads each operand tile once per block and
-// every operand M/128 times in total.  K is split into nsplit chunks;
-// chunk = blockIdx % nsplit keeps each chunk on one XCD.  A deterministic
-// second pass sums the chunks.
+// Three kernels: k_contract_f64_3m (the default for tables of at least
+// 128 x 128: three real products per complex product, 128 x 64 block tile
+// staged through LDS), k_contract_f64_lds (the four-product form with a
+// 128 x 128 tile, LQ_CONTRACT=4m) and k_contract_f64 (64 x 64 tile, operands
+// straight from global memory, for small tables).  The table is a GEMM with a
+// tiny output and a huge K: the LDS kernels read each operand tile once per
+// block, and K is split into nsplit chunks (chunk = blockIdx % nsplit keeps
+// each chunk on one XCD).  A deterministic second pass sums the chunks.
 #include <cstdlib>
 #include <cstring>
 

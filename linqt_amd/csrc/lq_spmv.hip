@@ -67,8 +67,9 @@ __device__ __forceinline__ int xcd_group(int b, int q) { return (b % kXcds) * q 
 // Entry accessors of the two operator forms, in two phases so that all
 // loads of a chunk's stored words are in flight before any value that depends
 // on them is fetched: load(p, w, a) reads stored slot p (plain: column and
-// value; dictionary: the packed word), decode(w, c, a) gives the column and
-// value (dictionary: looks the value up).  add_diag adds a separate diagonal.
+// value; dictionary: the packed word), column(w) gives the column and
+// value(w, a) the value (dictionary: looked up in the table).  diag() is the
+// separate diagonal of the dictionary form.
 struct PlainMat {
   const int *__restrict__ sptr;
   const int *__restrict__ col;
@@ -77,7 +78,8 @@ struct PlainMat {
     w = (unsigned)col[p];
     a = val[p];
   }
-  __device__ __forceinline__ void decode(unsigned w, int &c, double2 &) const { c = (int)w; }
+  __device__ __forceinline__ int column(unsigned w) const { return (int)w; }
+  __device__ __forceinline__ double2 value(unsigned, double2 a) const { return a; }
   __device__ __forceinline__ bool has_diag() const { return false; }
   __device__ __forceinline__ double2 diag(int) const { return make_double2(0.0, 0.0); }
 };
@@ -88,10 +90,9 @@ struct DictMat {
   const double2 *dict;  // value table (LDS copy, or global in the batch kernel)
   const double2 *__restrict__ dg;
   __device__ __forceinline__ void load(size_t p, unsigned &w, double2 &) const { w = wcol[p]; }
-  __device__ __forceinline__ void decode(unsigned w, int &c, double2 &a) const {
-    c = (int)(w & 0xffffffu);
-    a = dict[w >> 24];
-  }
+  __device__ __forceinline__ int column(unsigned w) const { return (int)(w & 0xffffffu); }
+  // the table is in LDS: read each value where it is used, not held in registers
+  __device__ __forceinline__ double2 value(unsigned w, double2) const { return dict[w >> 24]; }
   __device__ __forceinline__ bool has_diag() const { return dg != nullptr; }
   __device__ __forceinline__ double2 diag(int row) const { return dg[row]; }
 };
@@ -101,6 +102,21 @@ struct DictMat {
 // memory path that also carries the gathers.  The grids of these kernels are
 // capped (grid-stride) so the one-off staging per block is amortised.
 constexpr int kDictGridCap = 2048;
+
+// Schedule knobs (read once): LQ_SPMM_GRIDCAP overrides the dictionary-form
+// grid cap, LQ_SPMM_UNROLL (4, 8, 12, 16) the gather chunk.  Neither changes
+// the arithmetic of a row.
+int env_int(const char *name, int dflt) {
+  const char *e = getenv(name);
+  return e && *e ? atoi(e) : dflt;
+}
+int dict_grid_cap() {
+  static const int cap = [] {
+    const int c = env_int("LQ_SPMM_GRIDCAP", kDictGridCap);
+    return c >= 256 ? c : kDictGridCap;
+  }();
+  return cap;
+}
 
 __device__ __forceinline__ void stage(double2 *dst, const double2 *__restrict__ src, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
@@ -158,7 +174,6 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
       for (int j0 = 0; j0 < w; j0 += kUnroll) {
         unsigned wd[kUnroll];
         unsigned vw[NV > 0 ? kUnroll : 1];
-        int c[kUnroll];
         double2 a[kUnroll];
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
@@ -171,18 +186,19 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
           if (j0 + u < w) {
-            mat.decode(wd[u], c[u], a[u]);
-            xv[u] = x[(size_t)c[u] * kk + vv];
+            xv[u] = x[(size_t)mat.column(wd[u]) * kk + vv];
           } else {
             xv[u] = make_double2(0.0, 0.0);
           }
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-          ar = fma(a[u].x, xv[u].x, ar);
-          ar = fma(-a[u].y, xv[u].y, ar);
-          ai = fma(a[u].x, xv[u].y, ai);
-          ai = fma(a[u].y, xv[u].x, ai);
+          // padding slots: xv = 0, and a word of 0 reads table entry 0 (finite)
+          const double2 av = mat.value(wd[u], a[u]);
+          ar = fma(av.x, xv[u].x, ar);
+          ar = fma(-av.y, xv[u].y, ar);
+          ai = fma(av.x, xv[u].y, ai);
+          ai = fma(av.y, xv[u].x, ai);
         }
         if (vlane) {
 #pragma unroll
@@ -486,7 +502,7 @@ int launch_spmm_kpu(hipStream_t st, const SellDev &A, double2 alpha, double2 bet
   if (dot_mode != kDotNone) {
     grid = groups < kDotGridCap ? groups : kDotGridCap;
   } else if (DICT) {
-    grid = groups < kDictGridCap ? groups : kDictGridCap;
+    grid = groups < dict_grid_cap() ? groups : dict_grid_cap();
   } else {
     q = xcd_q(groups);
     if (q > 0) grid = q * kXcds;
@@ -535,7 +551,11 @@ int launch_spmm_kpu(hipStream_t st, const SellDev &A, double2 alpha, double2 bet
 
 // Unroll width: the smallest of 4/8/12/16 covering the widest slice (wider
 // operators loop over chunks of 16).
-int unroll_for(int wmax) { return wmax <= 4 ? 4 : wmax <= 8 ? 8 : wmax <= 12 ? 12 : 16; }
+int unroll_for(int wmax) {
+  static const int forced = env_int("LQ_SPMM_UNROLL", 0);
+  if (forced == 4 || forced == 8 || forced == 12 || forced == 16) return forced;
+  return wmax <= 4 ? 4 : wmax <= 8 ? 8 : wmax <= 12 ? 12 : 16;
+}
 
 template <int KP>
 int launch_spmm_kp(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, const SetArgs &sa,
@@ -543,7 +563,11 @@ int launch_spmm_kp(hipStream_t st, const SellDev &A, double2 alpha, double2 beta
 #define LQ_KPU(UV, DV) \
   return launch_spmm_kpu<KP, UV, DV>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used, fa)
   if (use_dict(A)) {
-    switch (unroll_for(A.dwmax)) {
+    // with fused right-operator applies a 12-entry chunk needs 164 VGPRs (3 waves
+    // per SIMD); 8-entry chunks need 128 (4 waves) and measured faster on rows
+    // of 9-12 entries despite the second chunk
+    const int u = unroll_for(A.dwmax);
+    switch (fa.nv > 0 && u == 12 && !getenv("LQ_SPMM_UNROLL") ? 8 : u) {
       case 4: LQ_KPU(4, true);
       case 8: LQ_KPU(8, true);
       case 12: LQ_KPU(12, true);

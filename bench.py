@@ -377,8 +377,9 @@ def main():
         for _ in range(reps):
             _lib.call("lq_memcpy_d2d", ctx.h, dst, src, nbytes)
         dt = (time.perf_counter() - t0) / reps
-        print(json.dumps({"calibrate": "d2d copy", "bytes_read": nbytes, "bytes_written": nbytes, "reps": reps,
-                          "ms": dt * 1e3, "GB/s (read+write)": 2 * nbytes / dt / 1e9}), flush=True)
+        out = {"calibrate": "d2d copy", "bytes_read": nbytes, "bytes_written": nbytes, "reps": reps,
+               "ms": dt * 1e3, "GB/s (read+write)": 2 * nbytes / dt / 1e9}
+        print(json.dumps(out), flush=True)
         return
 
     if args.workload == "ingest":

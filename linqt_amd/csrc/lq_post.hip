@@ -239,17 +239,27 @@ __device__ __forceinline__ double2 DgreenR_chebF(double x, double m) {
 }
 
 // spectralFunctionFromChebmom_FFTgrid.cpp:60-73.
-__global__ void k_dos_fftgrid(const double2 *__restrict__ mu, int M, int n, double W, double Ec, int num_div,
-                              double *__restrict__ E, double *__restrict__ rho) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= num_div) return;
+// One wavefront per energy: lane l sums the moments m = l, l + 64, ..., then
+// the 64 lane sums are added in a fixed tree (deterministic).
+__global__ __launch_bounds__(64) void k_dos_fftgrid(const double2 *__restrict__ mu, int M, int n, double W, double Ec,
+                                                    int num_div, double *__restrict__ E, double *__restrict__ rho) {
+  __shared__ double sh[64];
+  const int i = blockIdx.x;
+  const int l = threadIdx.x;
   const double a = W / 2.0;
   const double energ = cos(kPi * (i + 0.5) / num_div);
-  double out = 0;
-  for (int m0 = 0; m0 < M; ++m0) out += delta_chebF(energ, m0) * mu[m0].x;
-  out *= n / a;
-  E[i] = energ * a + Ec;
-  rho[i] = fabs(out);
+  double part = 0;
+  for (int m0 = l; m0 < M; m0 += 64) part += delta_chebF(energ, m0) * mu[m0].x;
+  sh[l] = part;
+  __syncthreads();
+  for (int h = 32; h > 0; h >>= 1) {
+    if (l < h) sh[l] += sh[l + h];
+    __syncthreads();
+  }
+  if (l == 0) {
+    E[i] = energ * a + Ec;
+    rho[i] = fabs(sh[0] * (n / a));
+  }
 }
 
 // kuboGreenwoodFromChebmom_FFTgrid.cpp:57-66 and kuboBastinFromChebmom_FFTgrid.cpp:77-86:
@@ -564,7 +574,7 @@ int launch_jackson_2d(hipStream_t st, const double2 *mu_in, int M1_in, double2 *
 
 int launch_dos_fftgrid(hipStream_t st, const double2 *mu, int M, int n, double W, double Ec, int num_div, double *E,
                        double *rho) {
-  hipLaunchKernelGGL(k_dos_fftgrid, dim3((num_div + 127) / 128), dim3(128), 0, st, mu, M, n, W, Ec, num_div, E, rho);
+  hipLaunchKernelGGL(k_dos_fftgrid, dim3(num_div), dim3(64), 0, st, mu, M, n, W, Ec, num_div, E, rho);
   LQ_HIP_CHECK(hipGetLastError());
   return LQ_OK;
 }

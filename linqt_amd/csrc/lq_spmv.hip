@@ -103,6 +103,33 @@ struct DictMat {
 // capped (grid-stride) so the one-off staging per block is amortised.
 constexpr int kDictGridCap = 2048;
 
+// Deterministic sum of the per-block dot partials of one slot, p[b * width + w]
+// over b < grid for each w < width (width a power of two <= 256), by a
+// 256-thread block: thread t takes w = t % width and the blocks
+// b = t / width, t / width + S, ... (S = 256 / width streams), coalesced; the S
+// stream sums of each w are then added pairwise in a fixed tree.
+__device__ void reduce_slot(const double2 *__restrict__ p, int grid, int width, double2 *__restrict__ out) {
+  __shared__ double2 sh[256];
+  const int t = threadIdx.x;
+  const int streams = 256 / width;
+  double sx = 0.0, sy = 0.0;
+  if (t < streams * width) {
+    const int w = t % width;
+    for (int b = t / width; b < grid; b += streams) {
+      const double2 v = p[(size_t)b * width + w];
+      sx += v.x;
+      sy += v.y;
+    }
+  }
+  sh[t] = make_double2(sx, sy);
+  __syncthreads();
+  for (int h = streams / 2; h > 0; h >>= 1) {
+    if (t < h * width) sh[t] = make_double2(sh[t].x + sh[t + h * width].x, sh[t].y + sh[t + h * width].y);
+    __syncthreads();
+  }
+  if (t < width) out[t] = sh[t];
+}
+
 // Schedule knobs (read once): LQ_SPMM_GRIDCAP overrides the dictionary-form
 // grid cap, LQ_SPMM_UNROLL (4, 8, 12, 16) the gather chunk.  Neither changes
 // the arithmetic of a row.
@@ -343,20 +370,14 @@ __global__ __launch_bounds__(256) void k_spmm_batch(const BatchJobDev *__restric
 }
 
 // mom[(s + 1) * KP + v] = sum_b partial[slot][b][v] for every step s a job ran
-// in this chunk (the same block order as k_reduce_partials).
-__global__ void k_batch_reduce(const BatchJobDev *__restrict__ jobs, const int *__restrict__ tc, int nslots, int kp) {
+// in this chunk (the same summation order as k_reduce_partials).
+__global__ __launch_bounds__(256) void k_batch_reduce(const BatchJobDev *__restrict__ jobs,
+                                                      const int *__restrict__ tc, int nslots, int kp) {
   const BatchJobDev &J = jobs[blockIdx.y];
   const int slot = blockIdx.x;
-  const int v = threadIdx.x;
   const int s = tc[0] + slot - J.t0;
-  if (s < 0 || s >= J.steps || v >= kp) return;
-  const double2 *p = J.partial + (size_t)slot * J.nblk * kp + v;
-  double sx = 0.0, sy = 0.0;
-  for (int b = 0; b < J.nblk; ++b) {
-    sx += p[(size_t)b * kp].x;
-    sy += p[(size_t)b * kp].y;
-  }
-  J.mom[(size_t)(s + 1) * kp + v] = make_double2(sx, sy);
+  if (s < 0 || s >= J.steps) return;  // uniform over the block
+  reduce_slot(J.partial + (size_t)slot * J.nblk * kp, J.nblk, kp, J.mom + (size_t)(s + 1) * kp);
 }
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
@@ -435,18 +456,12 @@ __global__ __launch_bounds__(256) void k_dot(int n, int k, const double2 *__rest
   block_reduce_store<KP, 1>(d, partial, blockIdx.x);
 }
 
-__global__ void k_reduce_partials(const double2 *__restrict__ partial, int nslots, int grid, int width,
-                                  double2 *__restrict__ out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nslots * width) return;
-  const int slot = t / width, w = t % width;
-  const double2 *p = partial + (size_t)slot * grid * width + w;
-  double sx = 0.0, sy = 0.0;
-  for (int b = 0; b < grid; ++b) {
-    sx += p[(size_t)b * width].x;
-    sy += p[(size_t)b * width].y;
-  }
-  out[t] = make_double2(sx, sy);
+// one block per slot: out[slot * width + w] = sum over b < grid of
+// partial[(slot * grid + b) * width + w], in the order of reduce_slot
+__global__ __launch_bounds__(256) void k_reduce_partials(const double2 *__restrict__ partial, int nslots, int grid,
+                                                         int width, double2 *__restrict__ out) {
+  const int slot = blockIdx.x;
+  reduce_slot(partial + (size_t)slot * grid * width, grid, width, out + (size_t)slot * width);
 }
 
 // dst[i*g + v] = src[(r0 + v)*n + i]: state-major -> interleaved block.
@@ -697,7 +712,11 @@ int launch_spmm_batch(hipStream_t st, int kp, int unroll, const BatchJobDev *job
 
 int launch_batch_reduce(hipStream_t st, const BatchJobDev *jobs, int njobs, const int *tc, int nslots, int kp) {
   if (njobs <= 0 || nslots <= 0) return LQ_OK;
-  hipLaunchKernelGGL(k_batch_reduce, dim3(nslots, njobs), dim3(kp < 64 ? 64 : kp), 0, st, jobs, tc, nslots, kp);
+  if (kp <= 0 || kp > 256 || (kp & (kp - 1))) {
+    set_error("launch_batch_reduce: width %d is not a power of two <= 256", kp);
+    return LQ_ERR_ARG;
+  }
+  hipLaunchKernelGGL(k_batch_reduce, dim3(nslots, njobs), dim3(256), 0, st, jobs, tc, nslots, kp);
   LQ_HIP_CHECK(hipGetLastError());
   return LQ_OK;
 }
@@ -734,10 +753,12 @@ int launch_dot(hipStream_t st, int n, int k, const double2 *a, const double2 *b,
 }
 
 int launch_reduce_partials(hipStream_t st, const double2 *partial, int nslots, int grid, int width, double2 *out) {
-  const int total = nslots * width;
-  if (total <= 0) return LQ_OK;
-  hipLaunchKernelGGL(k_reduce_partials, dim3((total + 255) / 256), dim3(256), 0, st, partial, nslots, grid, width,
-                     out);
+  if (nslots <= 0 || width <= 0) return LQ_OK;
+  if (width > 256 || (width & (width - 1))) {
+    set_error("launch_reduce_partials: width %d is not a power of two <= 256", width);
+    return LQ_ERR_ARG;
+  }
+  hipLaunchKernelGGL(k_reduce_partials, dim3(nslots), dim3(256), 0, st, partial, nslots, grid, width, out);
   LQ_HIP_CHECK(hipGetLastError());
   return LQ_OK;
 }

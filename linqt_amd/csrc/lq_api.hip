@@ -389,6 +389,7 @@ int upload_sell(lq_csr A) {
   d.nslices = nslices;
   d.stored = (long long)stored;
   d.wmax = wmax;
+  dict_schedule(d);
   LQ_HIP_CHECK(hipMalloc(&d.sptr, sizeof(int) * sptr.size()));
   LQ_HIP_CHECK(hipMalloc(&d.col, sizeof(int) * col.size()));
   LQ_HIP_CHECK(hipMalloc(&d.val, sizeof(double2) * val.size()));

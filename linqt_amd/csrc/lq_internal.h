@@ -62,7 +62,15 @@ struct SellDev {
   double2 *diag = nullptr;   // [n] (nullptr: zero diagonal)
   long long dstored = 0;
   int dwmax = 0;
+  // schedule of the dictionary-form launches, fixed when the operator is
+  // uploaded (dict_schedule): grid cap and prefetched row groups
+  int gridcap = 0;
+  bool pf = true;
 };
+
+// Reads the schedule knobs LQ_SPMM_GRIDCAP (grid cap of dictionary-form
+// launches, >= 256) and LQ_SPMM_PF (0: no prefetched row groups) into d.
+void dict_schedule(SellDev &d);
 
 // Arguments of one vector set of a (multi-set) SpMM launch: y = a*A*x + b*z
 // for k interleaved vectors ([n][k] layout).  z may alias y; z == nullptr

@@ -130,19 +130,13 @@ __device__ void reduce_slot(const double2 *__restrict__ p, int grid, int width, 
   if (t < width) out[t] = sh[t];
 }
 
-// Schedule knobs (read once): LQ_SPMM_GRIDCAP overrides the dictionary-form
-// grid cap, LQ_SPMM_UNROLL (4, 8, 12, 16) the gather chunk.  Neither changes
-// the arithmetic of a row.
+// Schedule knobs: LQ_SPMM_GRIDCAP overrides the dictionary-form grid cap and
+// LQ_SPMM_PF=0 turns the prefetched row groups off (both read when an
+// operator is uploaded), LQ_SPMM_UNROLL (4, 8, 12, 16; read once) sets the
+// gather chunk.  None of them changes the arithmetic of a row.
 int env_int(const char *name, int dflt) {
   const char *e = getenv(name);
   return e && *e ? atoi(e) : dflt;
-}
-int dict_grid_cap() {
-  static const int cap = [] {
-    const int c = env_int("LQ_SPMM_GRIDCAP", kDictGridCap);
-    return c >= 256 ? c : kDictGridCap;
-  }();
-  return cap;
 }
 
 __device__ __forceinline__ void stage(double2 *dst, const double2 *__restrict__ src, int n) {
@@ -153,11 +147,20 @@ __device__ __forceinline__ void stage(double2 *dst, const double2 *__restrict__ 
 // launch; dot contributions are accumulated into d.
 // NV > 0: the lanes of vector set fa.set also form yv[o] = V_o x (o < NV)
 // from the same gathers, V_o's values indexed per stored slot by fa.vidx.
-template <int KP, int DOT, int kUnroll, class Mat, int NV = 0>
+// PF (dictionary form, grid-stride launches): the block's slice offsets are
+// staged in LDS (sgeo[i] = {first stored slot, width} of its i-th row group)
+// and the packed words of the next row group's first chunk are loaded while
+// the current group's gathers are in flight, so an iteration waits on one
+// memory round trip instead of two.  The arithmetic is unchanged.
+constexpr int kPfGroups = 64;  // row groups per block covered by the staged offsets
+
+template <int KP, int DOT, int kUnroll, class Mat, int NV = 0, bool PF = false>
 __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, double2 beta, const SetArgs &sa,
                                           int nrowgroups, const double2 *__restrict__ phi, int g0, int gstep,
-                                          double2 (&d)[2], const FusedArgs &fa = FusedArgs{}) {
+                                          double2 (&d)[2], const FusedArgs &fa = FusedArgs{},
+                                          const int2 *sgeo = nullptr) {
   constexpr int RPB = 256 / KP;
+  static_assert(!PF || RPB <= 64, "staged offsets need row groups within one slice");
   const int tid = threadIdx.x;
   const int v = tid % KP;
   const int set = (v < sa.k[0]) ? 0 : 1;
@@ -168,12 +171,49 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
   const double2 *z = sa.z[set];
   double2 *y = sa.y[set];
   const bool vlane = NV > 0 && set == fa.set;
-  for (int g = g0; g < nrowgroups; g += gstep) {
+  // PF: slice offsets of the block's i-th row group from LDS, words of the next
+  // group's first chunk in pw
+  unsigned pw[PF ? kUnroll : 1];
+  auto geo = [&](int i, int gg, int &base_o, int &w_o) {
+    if (PF && i < kPfGroups) {
+      const int2 q = sgeo[i];
+      base_o = q.x;
+      w_o = q.y;
+    } else {
+      const int ss = (gg * RPB) >> 6;
+      base_o = mat.sptr[ss];
+      w_o = (mat.sptr[ss + 1] - base_o) >> 6;
+    }
+  };
+  auto fetch = [&](int i, int gg) {
+    if (!PF) return;
+    const int rr = gg * RPB + tid / KP;
+#pragma unroll
+    for (int u = 0; u < (PF ? kUnroll : 1); ++u) pw[u] = 0u;
+    if (gg < nrowgroups && rr < n && lane_on) {
+      int bb, ww;
+      geo(i, gg, bb, ww);
+      const size_t q0 = (size_t)bb + (rr & 63);
+      double2 dummy;
+#pragma unroll
+      for (int u = 0; u < (PF ? kUnroll : 1); ++u)
+        if (u < ww) mat.load(q0 + (size_t)u * kSlice, pw[u], dummy);
+    }
+  };
+  fetch(0, g0);
+  int it = 0;
+  for (int g = g0; g < nrowgroups; g += gstep, ++it) {
     const int row = g * RPB + tid / KP;
     if (row < n && lane_on) {
-      const int s = row >> 6, r = row & 63;
-      const int base = mat.sptr[s];
-      const int w = (mat.sptr[s + 1] - base) >> 6;
+      const int r = row & 63;
+      int base, w;
+      if (PF) {
+        geo(it, g, base, w);
+      } else {
+        const int s = row >> 6;
+        base = mat.sptr[s];
+        w = (mat.sptr[s + 1] - base) >> 6;
+      }
       const size_t p0 = (size_t)base + r;
       const size_t o = (size_t)row * kk + vv;
       // the separate diagonal (dictionary form) is loaded with the first
@@ -191,6 +231,8 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
       }
       if (mat.has_diag()) dv = mat.diag(row);
       if (mat.has_diag() || vdg) xo = x[o];
+      double2 zv = make_double2(0.0, 0.0);
+      if (PF && z) zv = z[o];
       double ar = 0.0, ai = 0.0;
       double vr[NV > 0 ? NV : 1], vi[NV > 0 ? NV : 1];
 #pragma unroll
@@ -206,7 +248,10 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
         for (int u = 0; u < kUnroll; ++u) {
           wd[u] = 0u;
           a[u] = make_double2(0.0, 0.0);
-          if (j0 + u < w) mat.load(p0 + (size_t)(j0 + u) * kSlice, wd[u], a[u]);
+          if (PF && j0 == 0)
+            wd[u] = pw[PF ? u : 0];
+          else if (j0 + u < w)
+            mat.load(p0 + (size_t)(j0 + u) * kSlice, wd[u], a[u]);
           if (NV > 0) vw[u] = (vlane && j0 + u < w) ? (unsigned)fa.vidx[p0 + (size_t)(j0 + u) * kSlice] : 0u;
         }
         double2 xv[kUnroll];
@@ -218,6 +263,7 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
             xv[u] = make_double2(0.0, 0.0);
           }
         }
+        if (PF && j0 == 0) fetch(it + 1, g + gstep);  // in flight behind this chunk's gathers
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
           // padding slots: xv = 0, and a word of 0 reads table entry 0 (finite)
@@ -243,6 +289,7 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
           }
         }
       }
+      if (PF && w <= 0) fetch(it + 1, g + gstep);
       if (vlane) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
@@ -261,7 +308,7 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
       }
       double2 res = make_double2(alpha.x * ar - alpha.y * ai, alpha.x * ai + alpha.y * ar);
       if (z) {
-        const double2 zv = z[o];
+        if (!PF) zv = z[o];
         res.x += beta.x * zv.x - beta.y * zv.y;
         res.y += beta.x * zv.y + beta.y * zv.x;
       }
@@ -294,7 +341,7 @@ struct MatArgs {
   int ndict;  // > 0: dictionary form
 };
 
-template <int KP, int DOT, int kUnroll, bool DICT, int NV = 0>
+template <int KP, int DOT, int kUnroll, bool DICT, int NV = 0, bool PF = false>
 __global__ __launch_bounds__(256) void k_spmm(int n, MatArgs m, double2 alpha, double2 beta, SetArgs sa,
                                               int nrowgroups, const double2 *__restrict__ phi,
                                               double2 *__restrict__ partial, int q, FusedArgs fa) {
@@ -307,6 +354,18 @@ __global__ __launch_bounds__(256) void k_spmm(int n, MatArgs m, double2 alpha, d
     __shared__ double2 sdict[256];
     __shared__ double2 svdict[NV > 0 ? NV : 1][NV > 0 ? 256 : 1];
     stage(sdict, m.dict, m.ndict);
+    __shared__ int2 sgeo[PF ? kPfGroups : 1];
+    if (PF) {
+      constexpr int RPB = 256 / KP;
+      for (int i = threadIdx.x; i < kPfGroups; i += blockDim.x) {
+        const int gg = g0 + i * gstep;
+        if (gg < nrowgroups) {
+          const int ss = (gg * RPB) >> 6;
+          const int b = m.sptr[ss];
+          sgeo[i] = make_int2(b, (m.sptr[ss + 1] - b) >> 6);
+        }
+      }
+    }
     FusedArgs fl = fa;
 #pragma unroll
     for (int o = 0; o < NV; ++o) {
@@ -315,7 +374,7 @@ __global__ __launch_bounds__(256) void k_spmm(int n, MatArgs m, double2 alpha, d
     }
     __syncthreads();
     const DictMat mat{m.sptr, m.wcol, sdict, m.diag};
-    spmm_body<KP, DOT, kUnroll, DictMat, NV>(n, mat, alpha, beta, sa, nrowgroups, phi, g0, gstep, d, fl);
+    spmm_body<KP, DOT, kUnroll, DictMat, NV, PF>(n, mat, alpha, beta, sa, nrowgroups, phi, g0, gstep, d, fl, sgeo);
   } else {
     const PlainMat mat{m.sptr, m.col, m.val};
     spmm_body<KP, DOT, kUnroll>(n, mat, alpha, beta, sa, nrowgroups, phi, g0, gstep, d);
@@ -330,7 +389,7 @@ __global__ __launch_bounds__(256) void k_spmm(int n, MatArgs m, double2 alpha, d
 // independent of what else is in the batch.
 //   s == 0:  T_1 = H~ T_0                     (x = buf0, y = buf1)
 //   s >= 1:  T_{s+1} = 2 H~ T_s - T_{s-1}     (in place over T_{s-1})
-template <int KP, int kUnroll>
+template <int KP, int kUnroll, bool PF = false>
 __global__ __launch_bounds__(256) void k_spmm_batch(const BatchJobDev *__restrict__ jobs, int njobs,
                                                     const int *__restrict__ tc, int slot) {
   int lo = 0, hi = njobs - 1;
@@ -359,9 +418,21 @@ __global__ __launch_bounds__(256) void k_spmm_batch(const BatchJobDev *__restric
   if (J.ndict > 0) {
     __shared__ double2 sdict[256];
     stage(sdict, J.dict, J.ndict);  // J is the same for the whole block
+    __shared__ int2 sgeo[PF ? kPfGroups : 1];
+    if (PF) {
+      for (int i = threadIdx.x; i < kPfGroups; i += blockDim.x) {
+        const int gg = lb + i * J.nblk;
+        if (gg < groups) {
+          const int ss = (gg * RPB) >> 6;
+          const int b = J.dsptr[ss];
+          sgeo[i] = make_int2(b, (J.dsptr[ss + 1] - b) >> 6);
+        }
+      }
+    }
     __syncthreads();
     const DictMat mat{J.dsptr, J.wcol, sdict, J.diag};
-    spmm_body<KP, kDotPhi, kUnroll>(J.n, mat, alpha, beta, sa, groups, J.phi, lb, J.nblk, d);
+    spmm_body<KP, kDotPhi, kUnroll, DictMat, 0, PF>(J.n, mat, alpha, beta, sa, groups, J.phi, lb, J.nblk, d,
+                                                    FusedArgs{}, sgeo);
   } else {
     const PlainMat mat{J.sptr, J.col, J.val};
     spmm_body<KP, kDotPhi, kUnroll>(J.n, mat, alpha, beta, sa, groups, J.phi, lb, J.nblk, d);
@@ -503,9 +574,40 @@ int xcd_q(int groups) { return knobs().xcd ? (groups + kXcds - 1) / kXcds : 0; }
 
 bool use_dict(const SellDev &A) { return A.ndict > 0; }
 
+}  // namespace
+
+void dict_schedule(SellDev &d) {
+  const int c = env_int("LQ_SPMM_GRIDCAP", kDictGridCap);
+  d.gridcap = c >= 256 ? c : kDictGridCap;
+  d.pf = env_int("LQ_SPMM_PF", 1) != 0;
+}
+
+namespace {
+
 MatArgs mat_args(const SellDev &A) {
   if (use_dict(A)) return MatArgs{A.dsptr, nullptr, nullptr, A.wcol, A.dict, A.diag, A.ndict};
   return MatArgs{A.sptr, A.col, A.val, nullptr, nullptr, nullptr, 0};
+}
+
+// Prefetched row-group schedule (dictionary form, up to 64 rows per group,
+// no fused operators): on unless LQ_SPMM_PF=0.  The fused kernel is left
+// without it: the prefetched words take it past 128 VGPRs, and losing a wave
+// per SIMD costs more than the round trip saved.
+bool spmm_pf() { return env_int("LQ_SPMM_PF", 1) != 0; }
+
+template <int KP, int DOT, int U, bool DICT, int NV>
+void launch_k_spmm(hipStream_t st, int grid, int n, const MatArgs &m, double2 alpha, double2 beta,
+                   const SetArgs &sa, int groups, const double2 *phi, double2 *partial, int q, const FusedArgs &fa,
+                   bool pf) {
+  if constexpr (DICT && NV == 0 && 256 / KP <= 64) {
+    if (q == 0 && pf) {
+      hipLaunchKernelGGL((k_spmm<KP, DOT, U, DICT, NV, true>), dim3(grid), dim3(256), 0, st, n, m, alpha, beta, sa,
+                         groups, phi, partial, q, fa);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_spmm<KP, DOT, U, DICT, NV>), dim3(grid), dim3(256), 0, st, n, m, alpha, beta, sa, groups,
+                     phi, partial, q, fa);
 }
 
 template <int KP, int U, bool DICT>
@@ -517,7 +619,7 @@ int launch_spmm_kpu(hipStream_t st, const SellDev &A, double2 alpha, double2 bet
   if (dot_mode != kDotNone) {
     grid = groups < kDotGridCap ? groups : kDotGridCap;
   } else if (DICT) {
-    grid = groups < dict_grid_cap() ? groups : dict_grid_cap();
+    grid = groups < A.gridcap ? groups : A.gridcap;
   } else {
     q = xcd_q(groups);
     if (q > 0) grid = q * kXcds;
@@ -531,11 +633,9 @@ int launch_spmm_kpu(hipStream_t st, const SellDev &A, double2 alpha, double2 bet
         return LQ_ERR_ARG;
       }
       if (fa.nv == 1)
-        hipLaunchKernelGGL((k_spmm<KP, kDotNone, U, true, 1>), dim3(grid), dim3(256), 0, st, A.n, m, alpha, beta,
-                           sa, groups, phi, partial, q, fa);
+        launch_k_spmm<KP, kDotNone, U, true, 1>(st, grid, A.n, m, alpha, beta, sa, groups, phi, partial, q, fa, A.pf);
       else
-        hipLaunchKernelGGL((k_spmm<KP, kDotNone, U, true, 2>), dim3(grid), dim3(256), 0, st, A.n, m, alpha, beta,
-                           sa, groups, phi, partial, q, fa);
+        launch_k_spmm<KP, kDotNone, U, true, 2>(st, grid, A.n, m, alpha, beta, sa, groups, phi, partial, q, fa, A.pf);
       LQ_HIP_CHECK(hipGetLastError());
       return LQ_OK;
     } else {
@@ -545,16 +645,13 @@ int launch_spmm_kpu(hipStream_t st, const SellDev &A, double2 alpha, double2 bet
   }
   switch (dot_mode) {
     case kDotNone:
-      hipLaunchKernelGGL((k_spmm<KP, kDotNone, U, DICT>), dim3(grid), dim3(256), 0, st, A.n, m, alpha, beta, sa,
-                         groups, phi, partial, q, fa);
+      launch_k_spmm<KP, kDotNone, U, DICT, 0>(st, grid, A.n, m, alpha, beta, sa, groups, phi, partial, q, fa, A.pf);
       break;
     case kDotPhi:
-      hipLaunchKernelGGL((k_spmm<KP, kDotPhi, U, DICT>), dim3(grid), dim3(256), 0, st, A.n, m, alpha, beta, sa,
-                         groups, phi, partial, 0, fa);
+      launch_k_spmm<KP, kDotPhi, U, DICT, 0>(st, grid, A.n, m, alpha, beta, sa, groups, phi, partial, 0, fa, A.pf);
       break;
     case kDotDoubling:
-      hipLaunchKernelGGL((k_spmm<KP, kDotDoubling, U, DICT>), dim3(grid), dim3(256), 0, st, A.n, m, alpha, beta, sa,
-                         groups, phi, partial, 0, fa);
+      launch_k_spmm<KP, kDotDoubling, U, DICT, 0>(st, grid, A.n, m, alpha, beta, sa, groups, phi, partial, 0, fa, A.pf);
       break;
     default:
       set_error("launch_spmm: bad dot mode %d", dot_mode);
@@ -692,7 +789,11 @@ int launch_spmm_batch(hipStream_t st, int kp, int unroll, const BatchJobDev *job
   if (njobs <= 0 || grid <= 0) return LQ_OK;
 #define LQ_BU(KPV, UV)                                                                                     \
   if (kp == KPV && unroll == UV) {                                                                       \
-    hipLaunchKernelGGL((k_spmm_batch<KPV, UV>), dim3(grid), dim3(256), 0, st, jobs, njobs, tc, slot); \
+    if (256 / KPV <= 64 && spmm_pf())                                                                    \
+      hipLaunchKernelGGL((k_spmm_batch<KPV, UV, (256 / KPV <= 64)>), dim3(grid), dim3(256), 0, st, jobs,  \
+                         njobs, tc, slot);                                                               \
+    else                                                                                                 \
+      hipLaunchKernelGGL((k_spmm_batch<KPV, UV>), dim3(grid), dim3(256), 0, st, jobs, njobs, tc, slot);  \
     LQ_HIP_CHECK(hipGetLastError());                                                                     \
     return LQ_OK;                                                                                        \
   }

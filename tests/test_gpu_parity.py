@@ -251,6 +251,55 @@ def test_dictionary_form_matches_plain_form(ctx, monkeypatch):
     assert rel_inf(lq.spectral_moments(D, None, phi, 200), lq.spectral_moments(P, None, phi, 200)) < 1e-12
 
 
+def test_prefetched_row_groups_bitwise_equal(ctx, monkeypatch):
+    """The prefetched row-group schedule of the dictionary form (slice offsets
+    staged per block, next group's words loaded behind the gathers) leaves
+    every result bit unchanged, also past the 64 staged groups of a block
+    (grid cap 256: 68 and 135 groups per block here) and over slices that
+    hold only diagonal entries (width 0)."""
+    n = 1_100_000
+    rows = np.arange(n)
+    hop = (rows < n - 1) & ~((rows >= 127) & (rows < 256))  # rows 128..255 isolated
+    onsite = np.where((rows >= 128) & (rows < 256), 0.5 + 0.01 * (rows % 7), 0.0)
+    cnt = 1 + hop.astype(np.int64)
+    rowptr = np.concatenate([[0], np.cumsum(cnt)])
+    col = np.empty(rowptr[-1], np.int64)
+    val = np.empty(rowptr[-1], complex)
+    col[rowptr[:-1]] = rows
+    val[rowptr[:-1]] = onsite
+    h = rows[hop]
+    col[rowptr[h] + 1] = h + 1
+    val[rowptr[h] + 1] = -1.0 + 0.25j
+    m = io.CSR(n, rowptr, col, val)
+    monkeypatch.setenv("LQ_SPMM_DICT", "1")
+    monkeypatch.setenv("LQ_SPMM_GRIDCAP", "256")
+    monkeypatch.setenv("LQ_SPMM_PF", "0")
+    P = lq.SparseMatrix.from_csr(ctx, m).Rescale(1 / 2.7, 0.0)
+    monkeypatch.setenv("LQ_SPMM_PF", "1")
+    F = lq.SparseMatrix.from_csr(ctx, m).Rescale(1 / 2.7, 0.0)
+    monkeypatch.delenv("LQ_SPMM_GRIDCAP")
+    D = lq.SparseMatrix.from_csr(ctx, m).Rescale(1 / 2.7, 0.0)
+    rng = np.random.default_rng(17)
+    for k in (3, 8):
+        say("prefetch schedule", n, k)
+        x = rng.normal(size=(n, k)) + 1j * rng.normal(size=(n, k))
+        y = rng.normal(size=(n, k)) + 1j * rng.normal(size=(n, k))
+        ref = P.Multiply(0.7 - 0.2j, x, -1.0, y)
+        assert np.array_equal(F.Multiply(0.7 - 0.2j, x, -1.0, y), ref)
+        assert np.array_equal(D.Multiply(0.7 - 0.2j, x, -1.0, y), ref)
+        # spot rows against the chain's own formula
+        for i in (0, 126, 127, 128, 200, 255, 256, 70_000, n - 1):
+            e = onsite[i] * x[i]
+            if hop[i]:
+                e = e + (-1.0 + 0.25j) * x[i + 1]
+            if i > 0 and hop[i - 1]:
+                e = e + (-1.0 - 0.25j) * x[i - 1]
+            assert np.allclose(ref[i], (0.7 - 0.2j) * e / 2.7 - y[i], rtol=0, atol=1e-13)
+        phi = lattices.random_phase_states(n, k, seed=k)
+        mu = lq.spectral_moments(P, None, phi, 24)
+        assert np.array_equal(lq.spectral_moments(F, None, phi, 24), mu)
+
+
 def test_fused_right_operators_match_separate_applies(ctx, oracle, monkeypatch):
     """V_x, V_y (patterns inside H's) are applied inside the H step; results
     equal the separate applies, the oracle, and an operator outside H's

@@ -47,13 +47,31 @@ __device__ __forceinline__ void mma_step(const double2 (&a)[2][4], const double2
       }
 }
 
+// Block -> (K chunk, output tile).  Blocks are dispatched round-robin over
+// the 8 XCDs (block b on XCD b % 8), each with its own L2.  xmap (nsplit a
+// multiple of 8): XCD x takes chunks x, x + 8, ... one after another, all
+// tiles of a chunk together, so every row of A and B in the chunk is read
+// from HBM once and then shared through that XCD's L2.  Otherwise
+// chunk = b % nsplit.  Either way each (chunk, tile) is one block and writes
+// the same partial table, so the result does not depend on the mapping.
+__device__ __forceinline__ void block_chunk_tile(int bid, int nsplit, int xmap, int &chunk, int &tile) {
+  if (xmap) {
+    const int tiles = (int)gridDim.x / nsplit;
+    const int j = bid >> 3;
+    chunk = (bid & 7) + 8 * (j / tiles);
+    tile = j % tiles;
+  } else {
+    chunk = bid % nsplit;
+    tile = bid / nsplit;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_contract_f64(const double2 *__restrict__ A, long long lda, int M0,
                                                       const double2 *__restrict__ B, long long ldb, int M1,
-                                                      long long K, long long kchunk, int nsplit, int tiles_n,
+                                                      long long K, long long kchunk, int nsplit, int tiles_n, int xmap,
                                                       double2 *__restrict__ work) {
-  const int bid = blockIdx.x;
-  const int chunk = bid % nsplit;
-  const int tile = bid / nsplit;
+  int chunk, tile;
+  block_chunk_tile(blockIdx.x, nsplit, xmap, chunk, tile);
   const int tm = tile / tiles_n, tn = tile % tiles_n;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int r16 = lane & 15, ks = lane >> 4;
@@ -150,13 +168,12 @@ constexpr int kLdsRow = kKC + 1;
 
 __global__ __launch_bounds__(512) void k_contract_f64_lds(const double2 *__restrict__ A, long long lda, int M0,
                                                           const double2 *__restrict__ B, long long ldb, int M1,
-                                                          long long K, long long kchunk, int nsplit, int tiles_n,
+                                                          long long K, long long kchunk, int nsplit, int tiles_n, int xmap,
                                                           double2 *__restrict__ work) {
   __shared__ double2 sA[2][kBT * kLdsRow];
   __shared__ double2 sB[2][kBT * kLdsRow];
-  const int bid = blockIdx.x;
-  const int chunk = bid % nsplit;
-  const int tile = bid / nsplit;
+  int chunk, tile;
+  block_chunk_tile(blockIdx.x, nsplit, xmap, chunk, tile);
   const int tm = tile / tiles_n, tn = tile % tiles_n;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -299,13 +316,12 @@ __device__ __forceinline__ void mma3_stage(const double2 *__restrict__ sA, const
 
 __global__ __launch_bounds__(512) void k_contract_f64_3m(const double2 *__restrict__ A, long long lda, int M0,
                                                          const double2 *__restrict__ B, long long ldb, int M1,
-                                                         long long K, long long kchunk, int nsplit, int tiles_n,
+                                                         long long K, long long kchunk, int nsplit, int tiles_n, int xmap,
                                                          double2 *__restrict__ work) {
   __shared__ double2 sA[2][kBM3 * kLdsRow];
   __shared__ double2 sB[2][kBN3 * kLdsRow];
-  const int bid = blockIdx.x;
-  const int chunk = bid % nsplit;
-  const int tile = bid / nsplit;
+  int chunk, tile;
+  block_chunk_tile(blockIdx.x, nsplit, xmap, chunk, tile);
   const int tm = tile / tiles_n, tn = tile % tiles_n;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -426,6 +442,16 @@ int contract_kind(int M0, int M1) {
 
 bool use_lds(int M0, int M1) { return contract_kind(M0, M1) != kWave; }
 
+// XCD-major block order (block_chunk_tile); LQ_CONTRACT_XMAP=0 restores
+// chunk = block % nsplit.
+bool contract_xmap() {
+  static const bool on = [] {
+    const char *e = getenv("LQ_CONTRACT_XMAP");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 int choose_nsplit(int M0, int M1, long long K) {
   const int kind = contract_kind(M0, M1);
   const int bm = kind == kWave ? 64 : kBT, bn = kind == kLds3m ? kBN3 : bm;
@@ -465,15 +491,16 @@ int launch_contract_f64(hipStream_t st, const double2 *A, long long lda, int M0,
     set_error("launch_contract_f64: grid too large");
     return LQ_ERR_ARG;
   }
+  const int xmap = (nsplit % 8 == 0 && contract_xmap()) ? 1 : 0;
   if (kind == kLds3m)
     hipLaunchKernelGGL(k_contract_f64_3m, dim3((unsigned)blocks), dim3(512), 0, st, A, lda, M0, B, ldb, M1, K,
-                       kchunk, nsplit, tiles_n, work);
+                       kchunk, nsplit, tiles_n, xmap, work);
   else if (kind == kLds4m)
     hipLaunchKernelGGL(k_contract_f64_lds, dim3((unsigned)blocks), dim3(512), 0, st, A, lda, M0, B, ldb, M1, K,
-                       kchunk, nsplit, tiles_n, work);
+                       kchunk, nsplit, tiles_n, xmap, work);
   else
     hipLaunchKernelGGL(k_contract_f64, dim3((unsigned)blocks), dim3(256), 0, st, A, lda, M0, B, ldb, M1, K, kchunk,
-                       nsplit, tiles_n, work);
+                       nsplit, tiles_n, xmap, work);
   LQ_HIP_CHECK(hipGetLastError());
   return LQ_OK;
 }

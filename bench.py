@@ -518,8 +518,10 @@ def main():
         ctr_kernel, ctr_fpc = "k_contract_f64_lds", 8
     elif ctr_mode == "wave" or config.get("M", 0) < 128:
         ctr_kernel, ctr_fpc = "k_contract_f64", 8
-    else:
+    elif os.environ.get("LQ_CONTRACT_DL", "") == "0":
         ctr_kernel, ctr_fpc = "k_contract_f64_3m", 6
+    else:
+        ctr_kernel, ctr_fpc = "k_contract_f64_3m_dl", 6
     ctr_tfs = ctr_cplx_tfs * ctr_fpc / 8
     kernels = {
         "spmv": {"ms_per_step": tm["spmv_ms"] / tsteps, "launches_per_step": tm["spmv_launches"] / tsteps,

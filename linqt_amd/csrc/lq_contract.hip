@@ -6,14 +6,17 @@
 // table is one complex GEMM  C = A^H B  on FP64 MFMA (v_mfma_f64_16x16x4_f64):
 //   Re C = Ar Br + Ai Bi,   Im C = Ar Bi - Ai Br.
 //
-// Three kernels: k_contract_f64_3m (the default for tables of at least
-// 128 x 128: three real products per complex product, 128 x 64 block tile
-// staged through LDS), k_contract_f64_lds (the four-product form with a
-// 128 x 128 tile, LQ_CONTRACT=4m) and k_contract_f64 (64 x 64 tile, operands
-// straight from global memory, for small tables).  The table is a GEMM with a
-// tiny output and a huge K: the LDS kernels read each operand tile once per
-// block, and K is split into nsplit chunks (chunk = blockIdx % nsplit keeps
-// each chunk on one XCD).  A deterministic second pass sums the chunks.
+// Kernels: k_contract_f64_3m_dl (the default for tables of at least
+// 128 x 128: three real products per complex product, 128 x 64 block tile,
+// operands copied from global memory straight into LDS buffers),
+// k_contract_f64_3m (the same products staged through registers into two LDS
+// buffers, LQ_CONTRACT_DL=0), k_contract_f64_lds (the four-product form with
+// a 128 x 128 tile, LQ_CONTRACT=4m) and k_contract_f64 (64 x 64 tile,
+// operands straight from global memory, for small tables).  The table is a
+// GEMM with a tiny output and a huge K: the LDS kernels read each operand tile
+// once per block, K is split into nsplit chunks, and the blocks of one chunk
+// run together on one XCD (block_chunk_tile).  A deterministic second pass
+// sums the chunks.
 #include <cstdlib>
 #include <cstring>
 
@@ -405,6 +408,191 @@ __global__ __launch_bounds__(512) void k_contract_f64_3m(const double2 *__restri
       }
 }
 
+// Direct-load variant of the three-multiplication kernel (same tiles, waves
+// and MFMA sequence): operands go from global memory straight into LDS with
+// global_load_lds_dwordx4 (lane l of a load writes 16 bytes at base + 16 l),
+// so no wave stages them through registers or stores them itself.  NBUF LDS
+// buffers (2 by default): the loads of stage st + NBUF - 1 are issued when
+// stage st starts, and one counter wait plus one barrier per stage make stage
+// st + 1 visible.  These loads are fetched from beyond L2 about twice as often
+// as the register-staged kernel's (measured; the cache-policy bits did not
+// change that), yet the kernel is faster: it is bound by the MFMA pipe.
+// LDS rows are 16 unpadded slots; element k of row R sits in slot
+// k ^ (R & 15), so the 16 rows read by one MFMA operand fetch hit 16
+// different slots and a 16-lane load group still covers one row's 256
+// contiguous bytes.  A partial last stage is loaded through registers with
+// zeros past K into the same slots.
+constexpr int kDlA = kBM3 * kKC, kDlB = kBN3 * kKC;  // slots per buffer
+
+__device__ __forceinline__ int dl_slot(int row, int k) { return row * kKC + (k ^ (row & 15)); }
+
+__device__ __forceinline__ void mma3_stage_dl(const double2 *sA, const double2 *sB, int arow0, int brow0, int ks,
+                                              v4d (&p1)[2][2], v4d (&p2)[2][2], v4d (&p3)[2][2]) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    double ar[2], ai[2], ad[2], br[2], bi[2], bs[2];
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti) {
+      const double2 x = sA[dl_slot(arow0 + ti * 16, ks * 4 + s)];
+      ar[ti] = x.x;
+      ai[ti] = x.y;
+      ad[ti] = x.x - x.y;
+    }
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) {
+      const double2 y = sB[dl_slot(brow0 + tj * 16, ks * 4 + s)];
+      br[tj] = y.x;
+      bi[tj] = y.y;
+      bs[tj] = y.x + y.y;
+    }
+#pragma unroll
+    for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj) {
+        p1[ti][tj] = mfma(ar[ti], br[tj], p1[ti][tj]);
+        p2[ti][tj] = mfma(ai[ti], bi[tj], p2[ti][tj]);
+        p3[ti][tj] = mfma(ad[ti], bs[tj], p3[ti][tj]);
+      }
+  }
+}
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// One global_load_lds_dwordx4: lane l copies 16 bytes from g (its own
+// address) to LDS byte address lds + 16 l (lds uniform over the wave).
+// Issued as inline assembly so the compiler does not add a wait for it before
+// unrelated LDS reads; its completion is awaited explicitly (s_waitcnt vmcnt).
+__device__ __forceinline__ void lds_dma16(const double2 *g, double2 *lds) {
+  const unsigned a = (unsigned)(size_t)(lds_void_t *)lds;
+  asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off"
+               :
+               : "s"(__builtin_amdgcn_readfirstlane(a)), "v"(g)
+               : "memory", "m0");
+}
+
+template <int NBUF>
+__global__ __launch_bounds__(512) void k_contract_f64_3m_dl(const double2 *__restrict__ A, long long lda, int M0,
+                                                            const double2 *__restrict__ B, long long ldb, int M1,
+                                                            long long K, long long kchunk, int nsplit, int tiles_n,
+                                                            int xmap, double2 *__restrict__ work) {
+  // three separate arrays (not one indexed by stage % 3) so the compiler can
+  // tell the buffer being read from the ones the loads in flight write
+  static_assert(NBUF == 2 || NBUF == 3, "two or three LDS buffers");
+  __shared__ double2 sA0[kDlA], sA1[kDlA], sA2[NBUF == 3 ? kDlA : 1];
+  __shared__ double2 sB0[kDlB], sB1[kDlB], sB2[NBUF == 3 ? kDlB : 1];
+  int chunk, tile;
+  block_chunk_tile(blockIdx.x, nsplit, xmap, chunk, tile);
+  const int tm = tile / tiles_n, tn = tile % tiles_n;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int r16 = lane & 15, ks = lane >> 4;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+
+  // Load q of this wave covers slots 64 c .. 64 c + 63 of a buffer (c = 4 wave + q
+  // for A, 2 wave + q for B): lane l fills slot 64 c + l = row R = 4 c + l / 16,
+  // element (l % 16) ^ (R & 15).
+  const int lrow = lane >> 4, lk = lane & 15;
+  const double2 *pa[4];
+  const double2 *pb[2];
+  int ka[4], kb2[2];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int R = 4 * (4 * wave + q) + lrow;
+    int g = tm * kBM3 + R;
+    if (g >= M0) g = M0 - 1;  // clamped rows feed outputs that are never stored
+    ka[q] = lk ^ (R & 15);
+    pa[q] = A + (size_t)g * lda + ka[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int R = 4 * (2 * wave + q) + lrow;
+    int g = tn * kBN3 + R;
+    if (g >= M1) g = M1 - 1;
+    kb2[q] = lk ^ (R & 15);
+    pb[q] = B + (size_t)g * ldb + kb2[q];
+  }
+
+  const long long k0 = (long long)chunk * kchunk;
+  const long long k1 = (k0 + kchunk < K) ? k0 + kchunk : K;
+  const int nstage = k1 > k0 ? (int)((k1 - k0 + kKC - 1) / kKC) : 0;
+
+  // Issue the loads of stage j into (sA, sB).  Whole stage in range (uniform
+  // over the block): six direct loads per lane.  Partial stage: through
+  // registers, zeros past k1.
+  auto issue = [&](int j, double2 *sA, double2 *sB) {
+    const long long kb = k0 + (long long)j * kKC;
+    if (kb + kKC <= k1) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        lds_dma16(pa[q] + kb, sA + 64 * (4 * wave + q));
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+        lds_dma16(pb[q] + kb, sB + 64 * (2 * wave + q));
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        sA[64 * (4 * wave + q) + lane] = kb + ka[q] < k1 ? pa[q][kb] : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+        sB[64 * (2 * wave + q) + lane] = kb + kb2[q] < k1 ? pb[q][kb] : make_double2(0.0, 0.0);
+    }
+  };
+  v4d p1[2][2], p2[2][2], p3[2][2];
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) p1[ti][tj] = p2[ti][tj] = p3[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  // After computing stage j: this wave's loads of stage j + 1 complete (with
+  // three buffers the six of stage j + 2 may still be in flight), then a
+  // barrier without a memory fence (a fence would also wait for stage j + 2).
+  // Every LDS read of stage j has returned (lgkmcnt) before the barrier, so its
+  // buffer is free again.
+  auto settle = [&](int j) {
+    if (NBUF == 3 && j + 2 < nstage)
+      asm volatile("s_waitcnt vmcnt(6) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    else
+      asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+  };
+  // Stage j from (cA, cB); first the loads of stage j + NBUF - 1 into (nA, nB).
+  auto step = [&](int j, const double2 *cA, const double2 *cB, double2 *nA, double2 *nB) {
+    if (j + NBUF - 1 < nstage) issue(j + NBUF - 1, nA, nB);
+    mma3_stage_dl(cA, cB, wm + r16, wn + r16, ks, p1, p2, p3);
+    settle(j);
+  };
+
+  if (nstage > 0) issue(0, sA0, sB0);
+  if (NBUF == 3 && nstage > 1) issue(1, sA1, sB1);
+  settle(-1);
+  if (NBUF == 3) {
+    for (int st = 0; st < nstage; st += 3) {
+      step(st, sA0, sB0, sA2, sB2);
+      if (st + 1 < nstage) step(st + 1, sA1, sB1, sA0, sB0);
+      if (st + 2 < nstage) step(st + 2, sA2, sB2, sA1, sB1);
+    }
+  } else {
+    for (int st = 0; st < nstage; st += 2) {
+      step(st, sA0, sB0, sA1, sB1);
+      if (st + 1 < nstage) step(st + 1, sA1, sB1, sA0, sB0);
+    }
+  }
+
+  // D layout of v_mfma_f64_16x16x4: element r of lane l is D[l/16 + 4*r][l % 16].
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = tm * kBM3 + wm + ti * 16 + ks + 4 * r;
+        const int nn = tn * kBN3 + wn + tj * 16 + r16;
+        if (m < M0 && nn < M1) {
+          const double a1 = p1[ti][tj][r], a2 = p2[ti][tj][r];
+          work[((size_t)chunk * M0 + m) * M1 + nn] = make_double2(a1 + a2, p3[ti][tj][r] - a1 + a2);
+        }
+      }
+}
+
 __global__ void k_contract_reduce(const double2 *__restrict__ work, int nsplit, int M0, int M1,
                                   double2 *__restrict__ out, int ldo, int accumulate) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -441,6 +629,28 @@ int contract_kind(int M0, int M1) {
 }
 
 bool use_lds(int M0, int M1) { return contract_kind(M0, M1) != kWave; }
+
+// Three-multiplication kernel with direct global -> LDS loads
+// (k_contract_f64_3m_dl, the default); LQ_CONTRACT_DL=0 selects the
+// register-staged k_contract_f64_3m.  Both compute the same products in the
+// same order.
+bool contract_dl() {
+  static const bool on = [] {
+    const char *e = getenv("LQ_CONTRACT_DL");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+// LDS buffers of the direct-load kernel: 2 (loads one stage ahead, 96 KB) or
+// 3 (LQ_CONTRACT_DLBUF=3: two stages ahead, 144 KB; measured 1-2 % slower).
+int contract_dlbuf() {
+  static const int nb = [] {
+    const char *e = getenv("LQ_CONTRACT_DLBUF");
+    return e && e[0] == '3' ? 3 : 2;
+  }();
+  return nb;
+}
 
 // XCD-major block order (block_chunk_tile); LQ_CONTRACT_XMAP=0 restores
 // chunk = block % nsplit.
@@ -492,7 +702,13 @@ int launch_contract_f64(hipStream_t st, const double2 *A, long long lda, int M0,
     return LQ_ERR_ARG;
   }
   const int xmap = (nsplit % 8 == 0 && contract_xmap()) ? 1 : 0;
-  if (kind == kLds3m)
+  if (kind == kLds3m && contract_dl() && contract_dlbuf() == 2)
+    hipLaunchKernelGGL(k_contract_f64_3m_dl<2>, dim3((unsigned)blocks), dim3(512), 0, st, A, lda, M0, B, ldb, M1, K,
+                       kchunk, nsplit, tiles_n, xmap, work);
+  else if (kind == kLds3m && contract_dl())
+    hipLaunchKernelGGL(k_contract_f64_3m_dl<3>, dim3((unsigned)blocks), dim3(512), 0, st, A, lda, M0, B, ldb, M1, K,
+                       kchunk, nsplit, tiles_n, xmap, work);
+  else if (kind == kLds3m)
     hipLaunchKernelGGL(k_contract_f64_3m, dim3((unsigned)blocks), dim3(512), 0, st, A, lda, M0, B, ldb, M1, K,
                        kchunk, nsplit, tiles_n, xmap, work);
   else if (kind == kLds4m)

@@ -84,6 +84,55 @@ def test_compute_mom_table(ctx, ML, MR, K):
     assert rel_inf(sub, ref) < 1e-13
 
 
+_TABLE_SCRIPT = r"""
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import linqt_amd as lq
+from linqt_amd import _lib
+ML, MR, K = (int(a) for a in sys.argv[2:5])
+rng = np.random.default_rng(5)
+L = rng.normal(size=(ML, K)) + 1j * rng.normal(size=(ML, K))
+R = rng.normal(size=(MR, K)) + 1j * rng.normal(size=(MR, K))
+ctx = lq.Context(0)
+pl, pr, ps = ctx.alloc(L.nbytes), ctx.alloc(R.nbytes), ctx.alloc(ML * MR * 16)
+ctx.upload(pl, L)
+ctx.upload(pr, R)
+_lib.call("lq_compute_mom_table", ctx.h, pl, K, ML, pr, K, MR, K, ps, MR)
+sub = np.zeros((ML, MR), dtype=np.complex128)
+ctx.download(ps, sub)
+np.save(sys.argv[5], sub)
+"""
+
+
+@pytest.mark.parametrize("ML,MR,K", [(320, 200, 20483), (384, 320, 65536)])
+def test_contraction_kernels_bitwise_equal(tmp_path, ML, MR, K):
+    """The three-product kernels -- direct-to-LDS loads with two buffers (the
+    default) or three (LQ_CONTRACT_DLBUF=3), and register-staged
+    (LQ_CONTRACT_DL=0) -- form the same products in the same order: equal
+    tables, bit for bit, including a partial last K stage and edge tiles.  The
+    kernel is chosen once per process, so each runs in a child process."""
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = {}
+    for tag, extra in (("dl2", {}), ("dl3", {"LQ_CONTRACT_DLBUF": "3"}), ("reg", {"LQ_CONTRACT_DL": "0"})):
+        say("contraction kernels", ML, MR, K, tag)
+        path = tmp_path / f"{tag}.npy"
+        env = {k: v for k, v in os.environ.items() if not k.startswith("LQ_CONTRACT")}
+        env.update(extra)
+        r = subprocess.run([sys.executable, "-c", _TABLE_SCRIPT, root, str(ML), str(MR), str(K), str(path)],
+                           env=env, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        out[tag] = np.load(path)
+    assert np.array_equal(out["dl2"], out["reg"])
+    assert np.array_equal(out["dl3"], out["reg"])
+    rng = np.random.default_rng(5)
+    L = rng.normal(size=(ML, K)) + 1j * rng.normal(size=(ML, K))
+    R = rng.normal(size=(MR, K)) + 1j * rng.normal(size=(MR, K))
+    assert rel_inf(out["dl2"], L.conj() @ R.T) < 1e-13
+
+
 @pytest.mark.parametrize("M,W,name", [
     (100, 10.0, "NonLEqOpVX-VXlinear_chainKPM_M100x100.chebmom2D"),
     (300, 4.9, "NonLEqOpVX-VXlinear_chainKPM_M300x300.chebmom2D"),

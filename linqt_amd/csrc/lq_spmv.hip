@@ -148,10 +148,12 @@ __device__ __forceinline__ void stage(double2 *dst, const double2 *__restrict__ 
 // NV > 0: the lanes of vector set fa.set also form yv[o] = V_o x (o < NV)
 // from the same gathers, V_o's values indexed per stored slot by fa.vidx.
 // PF (dictionary form, grid-stride launches): the block's slice offsets are
-// staged in LDS (sgeo[i] = {first stored slot, width} of its i-th row group)
-// and the packed words of the next row group's first chunk are loaded while
-// the current group's gathers are in flight, so an iteration waits on one
-// memory round trip instead of two.  The arithmetic is unchanged.
+// staged in LDS (sgeo[i] = {first stored slot, width} of its i-th row group);
+// the packed words of the next chunk (of this row, or the next row group's
+// first) are loaded while the current chunk's gathers are in flight, and the
+// diagonal, the row's own x and z are loaded together after the last chunk, so
+// a row group of two chunks waits on three memory round trips instead of six.
+// The arithmetic is unchanged.
 constexpr int kPfGroups = 64;  // row groups per block covered by the staged offsets
 
 template <int KP, int DOT, int kUnroll, class Mat, int NV = 0, bool PF = false>
@@ -171,9 +173,17 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
   const double2 *z = sa.z[set];
   double2 *y = sa.y[set];
   const bool vlane = NV > 0 && set == fa.set;
-  // PF: slice offsets of the block's i-th row group from LDS, words of the next
-  // group's first chunk in pw
+  // PF: slice offsets of the block's i-th row group from LDS; pw holds the
+  // packed words of the next chunk (of this row, or the next row group's first)
   unsigned pw[PF ? kUnroll : 1];
+  auto fetch_chunk = [&](size_t p, int cnt) {  // words of slots p, p + kSlice, ...
+#pragma unroll
+    for (int u = 0; u < (PF ? kUnroll : 1); ++u) {
+      pw[u] = 0u;
+      double2 dummy;
+      if (u < cnt) mat.load(p + (size_t)u * kSlice, pw[u], dummy);
+    }
+  };
   auto geo = [&](int i, int gg, int &base_o, int &w_o) {
     if (PF && i < kPfGroups) {
       const int2 q = sgeo[i];
@@ -185,20 +195,12 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
       w_o = (mat.sptr[ss + 1] - base_o) >> 6;
     }
   };
-  auto fetch = [&](int i, int gg) {
+  auto fetch = [&](int i, int gg) {  // first chunk of row group gg (the block's i-th)
     if (!PF) return;
     const int rr = gg * RPB + tid / KP;
-#pragma unroll
-    for (int u = 0; u < (PF ? kUnroll : 1); ++u) pw[u] = 0u;
-    if (gg < nrowgroups && rr < n && lane_on) {
-      int bb, ww;
-      geo(i, gg, bb, ww);
-      const size_t q0 = (size_t)bb + (rr & 63);
-      double2 dummy;
-#pragma unroll
-      for (int u = 0; u < (PF ? kUnroll : 1); ++u)
-        if (u < ww) mat.load(q0 + (size_t)u * kSlice, pw[u], dummy);
-    }
+    int bb = 0, ww = 0;
+    if (gg < nrowgroups && rr < n && lane_on) geo(i, gg, bb, ww);
+    fetch_chunk((size_t)bb + (rr & 63), ww);
   };
   fetch(0, g0);
   int it = 0;
@@ -216,23 +218,27 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
       }
       const size_t p0 = (size_t)base + r;
       const size_t o = (size_t)row * kk + vv;
-      // the separate diagonal (dictionary form) is loaded with the first
-      // entries so it adds no memory round trip of its own
+      // The separate diagonal (dictionary form) and the row's own x: without
+      // PF loaded with the first entries; with PF loaded after the last chunk
+      // together with z (one round trip, and no registers held through the
+      // chunks).  Either way they enter the sums after the off-diagonal terms.
       double2 dv = make_double2(0.0, 0.0), xo = make_double2(0.0, 0.0);
       double2 vdv[NV > 0 ? NV : 1];
-      bool vdg = false;
-#pragma unroll
-      for (int q = 0; q < NV; ++q) {
-        vdv[q] = make_double2(0.0, 0.0);
-        if (vlane && fa.vdiag[q]) {
-          vdv[q] = fa.vdiag[q][row];
-          vdg = true;
-        }
-      }
-      if (mat.has_diag()) dv = mat.diag(row);
-      if (mat.has_diag() || vdg) xo = x[o];
       double2 zv = make_double2(0.0, 0.0);
-      if (PF && z) zv = z[o];
+      auto own_loads = [&]() {
+        bool vdg = false;
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+          vdv[q] = make_double2(0.0, 0.0);
+          if (vlane && fa.vdiag[q]) {
+            vdv[q] = fa.vdiag[q][row];
+            vdg = true;
+          }
+        }
+        if (mat.has_diag()) dv = mat.diag(row);
+        if (mat.has_diag() || vdg) xo = x[o];
+      };
+      if (!PF) own_loads();
       double ar = 0.0, ai = 0.0;
       double vr[NV > 0 ? NV : 1], vi[NV > 0 ? NV : 1];
 #pragma unroll
@@ -248,11 +254,11 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
         for (int u = 0; u < kUnroll; ++u) {
           wd[u] = 0u;
           a[u] = make_double2(0.0, 0.0);
-          if (PF && j0 == 0)
+          if (PF)
             wd[u] = pw[PF ? u : 0];
           else if (j0 + u < w)
             mat.load(p0 + (size_t)(j0 + u) * kSlice, wd[u], a[u]);
-          if (NV > 0) vw[u] = (vlane && j0 + u < w) ? (unsigned)fa.vidx[p0 + (size_t)(j0 + u) * kSlice] : 0u;
+          if (NV > 0 && !PF) vw[u] = (vlane && j0 + u < w) ? (unsigned)fa.vidx[p0 + (size_t)(j0 + u) * kSlice] : 0u;
         }
         double2 xv[kUnroll];
 #pragma unroll
@@ -263,7 +269,17 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
             xv[u] = make_double2(0.0, 0.0);
           }
         }
-        if (PF && j0 == 0) fetch(it + 1, g + gstep);  // in flight behind this chunk's gathers
+        if (PF) {
+          // behind this chunk's gathers: its value indices (fused operators)
+          // and the next chunk's words
+#pragma unroll
+          for (int u = 0; u < (NV > 0 ? kUnroll : 0); ++u)
+            vw[u] = (vlane && j0 + u < w) ? (unsigned)fa.vidx[p0 + (size_t)(j0 + u) * kSlice] : 0u;
+          if (j0 + kUnroll < w)
+            fetch_chunk(p0 + (size_t)(j0 + kUnroll) * kSlice, w - j0 - kUnroll);
+          else
+            fetch(it + 1, g + gstep);
+        }
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
           // padding slots: xv = 0, and a word of 0 reads table entry 0 (finite)
@@ -290,6 +306,10 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
         }
       }
       if (PF && w <= 0) fetch(it + 1, g + gstep);
+      if (PF) {
+        own_loads();
+        if (z) zv = z[o];
+      }
       if (vlane) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
@@ -589,10 +609,10 @@ MatArgs mat_args(const SellDev &A) {
   return MatArgs{A.sptr, A.col, A.val, nullptr, nullptr, nullptr, 0};
 }
 
-// Prefetched row-group schedule (dictionary form, up to 64 rows per group,
-// no fused operators): on unless LQ_SPMM_PF=0.  The fused kernel is left
-// without it: the prefetched words take it past 128 VGPRs, and losing a wave
-// per SIMD costs more than the round trip saved.
+// Prefetched schedule (dictionary form, up to 64 rows per group, no fused
+// operators): on unless LQ_SPMM_PF=0.  The fused kernel fits it in 122
+// registers too, but measured 1 % slower with it: with its fourteen read and
+// write streams it is bound by the HBM traffic, not by the round trips.
 bool spmm_pf() { return env_int("LQ_SPMM_PF", 1) != 0; }
 
 template <int KP, int DOT, int U, bool DICT, int NV>

@@ -7,8 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
-
-#include <map>
+#include <vector>
 
 #include "lq_internal.h"
 
@@ -138,6 +137,40 @@ void free_fused(FusedDev &f) {
   f = FusedDev{};
 }
 
+// Value table of the dictionary form: index of a complex value in first-seen
+// order, at most 256 values.  Open addressing over the bit patterns of
+// (re, im) with +0 and -0 taken as equal (values compare as numbers); the
+// table keeps the first-seen value.
+struct DictIndex {
+  static constexpr int kSlots = 1024;  // load factor <= 1/4
+  unsigned long long kre[kSlots], kim[kSlots];
+  short id[kSlots];
+  std::vector<double2> dict;
+  DictIndex() { std::fill(id, id + kSlots, (short)-1); }
+  static unsigned long long bits(double v) {
+    if (v == 0.0) v = 0.0;  // -0 -> +0
+    unsigned long long b;
+    std::memcpy(&b, &v, sizeof b);
+    return b;
+  }
+  // index of (re, im), added if new; -1 when a 257th value would be needed
+  int find_or_add(double re, double im) {
+    const unsigned long long a = bits(re), b = bits(im);
+    unsigned h = (unsigned)((a * 0x9E3779B97F4A7C15ull ^ b * 0xC2B2AE3D27D4EB4Full) >> 54);
+    for (;; h = (h + 1) & (kSlots - 1)) {
+      if (id[h] < 0) {
+        if (dict.size() == 256) return -1;
+        kre[h] = a;
+        kim[h] = b;
+        id[h] = (short)dict.size();
+        dict.push_back(d2(re, im));
+        return id[h];
+      }
+      if (kre[h] == a && kim[h] == b) return id[h];
+    }
+  }
+};
+
 // Aligns up to two right operators to the stored slots of H's dictionary
 // form (see FusedDev).  *ok = false (nothing built) when H has no dictionary
 // form, an operator has an off-diagonal entry outside H's pattern, or more
@@ -200,21 +233,13 @@ int fused_prepare(lq_csr H, const lq_csr *V, int nv, bool *ok) {
         sv[(size_t)sptr[sl] + (size_t)it->second * kSlice + r] += A->val[q];
       }
     }
-    std::map<std::pair<double, double>, int> index;
+    DictIndex index;
     for (size_t p = 0; p < stored; ++p) {
-      const auto key = std::make_pair(sv[p].real(), sv[p].imag());
-      auto it = index.find(key);
-      int id;
-      if (it == index.end()) {
-        if (index.size() == 256) return LQ_OK;
-        id = (int)index.size();
-        index.emplace(key, id);
-        dicts[o].push_back(d2(key.first, key.second));
-      } else {
-        id = it->second;
-      }
+      const int id = index.find_or_add(sv[p].real(), sv[p].imag());
+      if (id < 0) return LQ_OK;
       vidx[p] |= (unsigned short)(id << (8 * o));
     }
+    dicts[o] = std::move(index.dict);
     if (dicts[o].empty()) dicts[o].push_back(d2(0.0, 0.0));
     diags[o] = std::move(dg);
   }
@@ -258,11 +283,13 @@ int build_dict_form(const lq_csr A, SellDev &d) {
   const char *env = getenv("LQ_SPMM_DICT");
   if (env && env[0] == '0') return LQ_OK;
   const int nslices = (n + kSlice - 1) / kSlice;
-  std::map<std::pair<double, double>, int> index;
-  std::vector<double2> dict;
+  DictIndex index;
+  std::vector<unsigned char> eid(A->col.size());  // table index of each off-diagonal entry
   std::vector<double2> diag(n, d2(0.0, 0.0));
   bool has_diag = false;
   std::vector<int> offd(n, 0);
+  double lre = 0.0, lim = 0.0;  // last value looked up (runs of equal values are common)
+  int lid = -1;
   for (int i = 0; i < n; ++i)
     for (int q = A->rowptr[i]; q < A->rowptr[i + 1]; ++q) {
       if (A->col[q] == i) {
@@ -272,12 +299,14 @@ int build_dict_form(const lq_csr A, SellDev &d) {
         continue;
       }
       ++offd[i];
-      const auto key = std::make_pair(A->val[q].real(), A->val[q].imag());
-      if (!index.count(key)) {
-        if (dict.size() == 256) return LQ_OK;
-        index.emplace(key, (int)dict.size());
-        dict.push_back(d2(key.first, key.second));
+      const double re = A->val[q].real(), im = A->val[q].imag();
+      if (lid < 0 || !(re == lre && im == lim)) {
+        lid = index.find_or_add(re, im);
+        if (lid < 0) return LQ_OK;
+        lre = re;
+        lim = im;
       }
+      eid[q] = (unsigned char)lid;
     }
   std::vector<int> sptr(nslices + 1, 0);
   int wmax = 0;
@@ -298,14 +327,10 @@ int build_dict_form(const lq_csr A, SellDev &d) {
   }
   int zero = -1;
   if (padded) {
-    const auto key = std::make_pair(0.0, 0.0);
-    if (!index.count(key)) {
-      if (dict.size() == 256) return LQ_OK;
-      index.emplace(key, (int)dict.size());
-      dict.push_back(d2(0.0, 0.0));
-    }
-    zero = index[key];
+    zero = index.find_or_add(0.0, 0.0);
+    if (zero < 0) return LQ_OK;
   }
+  std::vector<double2> &dict = index.dict;
   const size_t stored = (size_t)sptr[nslices];
   std::vector<unsigned> wcol(stored > 0 ? stored : 1, 0u);
   for (int s = 0; s < nslices; ++s) {
@@ -316,8 +341,7 @@ int build_dict_form(const lq_csr A, SellDev &d) {
       if (i < n)
         for (int q = A->rowptr[i]; q < A->rowptr[i + 1]; ++q) {
           if (A->col[q] == i) continue;
-          const int id = index[std::make_pair(A->val[q].real(), A->val[q].imag())];
-          wcol[(size_t)sptr[s] + (size_t)j * kSlice + r] = (unsigned)A->col[q] | ((unsigned)id << 24);
+          wcol[(size_t)sptr[s] + (size_t)j * kSlice + r] = (unsigned)A->col[q] | ((unsigned)eid[q] << 24);
           ++j;
         }
       for (; j < w; ++j)

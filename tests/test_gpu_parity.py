@@ -111,12 +111,15 @@ def test_contraction_kernels_bitwise_equal(tmp_path, ML, MR, K):
     default) or three (LQ_CONTRACT_DLBUF=3), and register-staged
     (LQ_CONTRACT_DL=0) -- form the same products in the same order: equal
     tables, bit for bit, including a partial last K stage and edge tiles.  The
-    kernel is chosen once per process, so each runs in a child process."""
+    four-product kernel (LQ_CONTRACT=4m) rounds differently and is held to the
+    same tolerance against the product itself.  The kernel is chosen once per
+    process, so each runs in a child process."""
     import subprocess
 
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     out = {}
-    for tag, extra in (("dl2", {}), ("dl3", {"LQ_CONTRACT_DLBUF": "3"}), ("reg", {"LQ_CONTRACT_DL": "0"})):
+    for tag, extra in (("dl2", {}), ("dl3", {"LQ_CONTRACT_DLBUF": "3"}), ("reg", {"LQ_CONTRACT_DL": "0"}),
+                       ("4m", {"LQ_CONTRACT": "4m"})):
         say("contraction kernels", ML, MR, K, tag)
         path = tmp_path / f"{tag}.npy"
         env = {k: v for k, v in os.environ.items() if not k.startswith("LQ_CONTRACT")}
@@ -130,7 +133,9 @@ def test_contraction_kernels_bitwise_equal(tmp_path, ML, MR, K):
     rng = np.random.default_rng(5)
     L = rng.normal(size=(ML, K)) + 1j * rng.normal(size=(ML, K))
     R = rng.normal(size=(MR, K)) + 1j * rng.normal(size=(MR, K))
-    assert rel_inf(out["dl2"], L.conj() @ R.T) < 1e-13
+    ref = L.conj() @ R.T
+    assert rel_inf(out["dl2"], ref) < 1e-13
+    assert rel_inf(out["4m"], ref) < 1e-13
 
 
 @pytest.mark.parametrize("M,W,name", [

@@ -541,6 +541,13 @@ def main():
     except (OSError, ValueError, KeyError):
         traffic_tab = {}
     spmm_names = sorted((k for k in traffic_tab if k.startswith("k_spmm<")), key=lambda k: -traffic_tab[k])
+
+    def traffic_of(name):
+        # the summary names template kernels with their arguments (k_x<2>)
+        if name in traffic_tab:
+            return traffic_tab[name]
+        hits = [v for k, v in traffic_tab.items() if k.startswith(name + "<")]
+        return hits[0] if len(hits) == 1 else None
     sequential = None
     if args.workload == "batch":
         # the same jobs, one after another through the single-job solver (untimed in value)
@@ -572,7 +579,7 @@ def main():
         kname = f"k_contract_i8<{S}, {2 if S <= 5 else 1}>"
         roofline = {"bound": "mfma", "kernel": kname + " (+ k_slice)", "achieved": tops,
                     "peak": INT8_MFMA_PEAK_TOPS, "unit": "TOP/s", "frac": tops / INT8_MFMA_PEAK_TOPS,
-                    "traffic": traffic_tab.get(kname),
+                    "traffic": traffic_of(kname),
                     "per_launch": f"{ops:.4g} int8 op (8 M^2 ceil(N R / 4096) 4096 S(S+1)/2)",
                     "avg_launch_ms": avg_ms}
     elif tm["contract_ms"] > tm["spmv_ms"]:
@@ -581,7 +588,7 @@ def main():
         how = ("6*M^2*N*R: three real FP64 MFMA products per complex multiply-add" if ctr_fpc == 6
                else "8*M^2*N*R: four real FP64 MFMA products per complex multiply-add")
         roofline = {"bound": "mfma", "kernel": kname, "achieved": ctr_tfs, "peak": FP64_MFMA_PEAK_TFS,
-                    "unit": "TFLOP/s", "frac": ctr_tfs / FP64_MFMA_PEAK_TFS, "traffic": traffic_tab.get(kname),
+                    "unit": "TFLOP/s", "frac": ctr_tfs / FP64_MFMA_PEAK_TFS, "traffic": traffic_of(kname),
                     "per_launch": f"{per_launch:.4g} flop ({how})",
                     "complex_equiv_TFLOP/s": ctr_cplx_tfs,
                     "avg_launch_ms": tm["contract_ms"] / max(tm["contract_launches"], 1)}

@@ -113,6 +113,7 @@ struct FusedArgs {
   int nvdict[2] = {0, 0};
   const double2 *vdiag[2] = {nullptr, nullptr};
   double2 *yv[2] = {nullptr, nullptr};
+  int nt = 0;  // nonzero: yv stored and z loaded non-temporally (set by launch_spmm)
 };
 
 // One job of a batched recursion launch (device table entry).

@@ -139,6 +139,17 @@ int env_int(const char *name, int dflt) {
   return e && *e ? atoi(e) : dflt;
 }
 
+// Non-temporal 16-byte load and store (streamed past the caches).
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 nt_load2(const double2 *p) {
+  const dvec2 v = __builtin_nontemporal_load((const dvec2 *)p);
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void nt_store2(double2 *p, double2 a) {
+  const dvec2 v = {a.x, a.y};
+  __builtin_nontemporal_store(v, (dvec2 *)p);
+}
+
 __device__ __forceinline__ void stage(double2 *dst, const double2 *__restrict__ src, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
@@ -317,7 +328,10 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
           vr[q] = fma(-vdv[q].y, xo.y, vr[q]);
           vi[q] = fma(vdv[q].x, xo.y, vi[q]);
           vi[q] = fma(vdv[q].y, xo.x, vi[q]);
-          fa.yv[q][o] = make_double2(vr[q], vi[q]);
+          if (fa.nt)
+            nt_store2(fa.yv[q] + o, make_double2(vr[q], vi[q]));
+          else
+            fa.yv[q][o] = make_double2(vr[q], vi[q]);
         }
       }
       if (mat.has_diag()) {
@@ -328,7 +342,7 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
       }
       double2 res = make_double2(alpha.x * ar - alpha.y * ai, alpha.x * ai + alpha.y * ar);
       if (z) {
-        if (!PF) zv = z[o];
+        if (!PF) zv = (NV > 0 && fa.nt) ? nt_load2(z + o) : z[o];
         res.x += beta.x * zv.x - beta.y * zv.y;
         res.y += beta.x * zv.y + beta.y * zv.x;
       }
@@ -725,7 +739,14 @@ int spmm_dot_grid(int n, int kpad) {
 
 int launch_spmm(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, const VecSet *sets, int nsets,
                 int dot_mode, const double2 *phi, double2 *partial, int *grid_used, const FusedArgs *fused) {
-  const FusedArgs fa = fused ? *fused : FusedArgs{};
+  // Fused steps store the operator products (read again only by the
+  // contraction) and load T_{n-1} (read once) non-temporally, which leaves the
+  // caches to the vectors the next step gathers: 5 % faster on the Bastin step.
+  // LQ_SPMM_NT=0 turns it off.  T_{n+1} itself is gathered by the next step and
+  // measured 4 % slower when stored the same way.
+  FusedArgs fa = fused ? *fused : FusedArgs{};
+  static const int nt = env_int("LQ_SPMM_NT", 1);
+  fa.nt = fa.nv > 0 ? nt : 0;
   if (fa.nv < 0 || fa.nv > 2 || (fa.nv > 0 && (fa.set < 0 || fa.set >= nsets || !fa.vidx))) {
     set_error("launch_spmm: bad fused operator arguments");
     return LQ_ERR_ARG;

@@ -138,6 +138,46 @@ def test_contraction_kernels_bitwise_equal(tmp_path, ML, MR, K):
     assert rel_inf(out["4m"], ref) < 1e-13
 
 
+_FUSED_SCRIPT = r"""
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import linqt_amd as lq
+from linqt_amd import lattices
+ctx = lq.Context(0)
+H, Vx, Vy = lattices.haldane_honeycomb(24, seed=9)
+Hg = lq.SparseMatrix.from_csr(ctx, H).Rescale(1 / 4.6, 0.0)
+V = [lq.SparseMatrix.from_csr(ctx, Vx), lq.SparseMatrix.from_csr(ctx, Vy)]
+phi = lattices.random_phase_states(H.n, 3, seed=4)
+mu = lq.correlation_expansion_moments(Hg, V[0], V, phi, 96, 80)
+r = lq.kubo_fft(Hg, V[0], V, phi, 64, 256, lq.LQ_KUBO_BASTIN)
+np.savez(sys.argv[2], mu=mu, r=r)
+"""
+
+
+def test_nontemporal_streams_bitwise_equal(tmp_path):
+    """The fused recursion step stores the operator products and loads T_{n-1}
+    non-temporally unless LQ_SPMM_NT=0; only the cache policy differs, so
+    correlation tables and Kubo-Bastin sums are equal bit for bit.  The switch
+    is read once per process: each setting runs in a child process."""
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = {}
+    for nt in ("1", "0"):
+        say("non-temporal streams", "LQ_SPMM_NT=" + nt)
+        path = tmp_path / f"nt{nt}.npz"
+        env = {k: v for k, v in os.environ.items() if not k.startswith("LQ_SPMM")}
+        env["LQ_SPMM_NT"] = nt
+        r = subprocess.run([sys.executable, "-c", _FUSED_SCRIPT, root, str(path)], env=env, capture_output=True,
+                           text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        out[nt] = np.load(path)
+    assert np.array_equal(out["1"]["mu"], out["0"]["mu"])
+    assert np.array_equal(out["1"]["r"], out["0"]["r"])
+    assert np.all(np.isfinite(out["1"]["mu"])) and np.abs(out["1"]["mu"]).max() > 0
+
+
 @pytest.mark.parametrize("M,W,name", [
     (100, 10.0, "NonLEqOpVX-VXlinear_chainKPM_M100x100.chebmom2D"),
     (300, 4.9, "NonLEqOpVX-VXlinear_chainKPM_M300x300.chebmom2D"),

@@ -319,7 +319,7 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
       if (PF && w <= 0) fetch(it + 1, g + gstep);
       if (PF) {
         own_loads();
-        if (z) zv = z[o];
+        if (z) zv = fa.nt ? nt_load2(z + o) : z[o];
       }
       if (vlane) {
 #pragma unroll
@@ -342,7 +342,7 @@ __device__ __forceinline__ void spmm_body(int n, const Mat &mat, double2 alpha, 
       }
       double2 res = make_double2(alpha.x * ar - alpha.y * ai, alpha.x * ai + alpha.y * ar);
       if (z) {
-        if (!PF) zv = (NV > 0 && fa.nt) ? nt_load2(z + o) : z[o];
+        if (!PF) zv = fa.nt ? nt_load2(z + o) : z[o];
         res.x += beta.x * zv.x - beta.y * zv.y;
         res.y += beta.x * zv.y + beta.y * zv.x;
       }
@@ -425,7 +425,7 @@ __global__ __launch_bounds__(256) void k_spmm(int n, MatArgs m, double2 alpha, d
 //   s >= 1:  T_{s+1} = 2 H~ T_s - T_{s-1}     (in place over T_{s-1})
 template <int KP, int kUnroll, bool PF = false>
 __global__ __launch_bounds__(256) void k_spmm_batch(const BatchJobDev *__restrict__ jobs, int njobs,
-                                                    const int *__restrict__ tc, int slot) {
+                                                    const int *__restrict__ tc, int slot, int ntz) {
   int lo = 0, hi = njobs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -465,11 +465,15 @@ __global__ __launch_bounds__(256) void k_spmm_batch(const BatchJobDev *__restric
     }
     __syncthreads();
     const DictMat mat{J.dsptr, J.wcol, sdict, J.diag};
-    spmm_body<KP, kDotPhi, kUnroll, DictMat, 0, PF>(J.n, mat, alpha, beta, sa, groups, J.phi, lb, J.nblk, d,
-                                                    FusedArgs{}, sgeo);
+    FusedArgs bf;
+    bf.nt = ntz;
+    spmm_body<KP, kDotPhi, kUnroll, DictMat, 0, PF>(J.n, mat, alpha, beta, sa, groups, J.phi, lb, J.nblk, d, bf,
+                                                    sgeo);
   } else {
     const PlainMat mat{J.sptr, J.col, J.val};
-    spmm_body<KP, kDotPhi, kUnroll>(J.n, mat, alpha, beta, sa, groups, J.phi, lb, J.nblk, d);
+    FusedArgs bf;
+    bf.nt = ntz;
+    spmm_body<KP, kDotPhi, kUnroll>(J.n, mat, alpha, beta, sa, groups, J.phi, lb, J.nblk, d, bf);
   }
   block_reduce_store<KP, 1>(d, J.partial + (size_t)slot * J.nblk * KP, lb);
 }
@@ -742,11 +746,13 @@ int launch_spmm(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, c
   // Fused steps store the operator products (read again only by the
   // contraction) and load T_{n-1} (read once) non-temporally, which leaves the
   // caches to the vectors the next step gathers: 5 % faster on the Bastin step.
-  // LQ_SPMM_NT=0 turns it off.  T_{n+1} itself is gathered by the next step and
-  // measured 4 % slower when stored the same way.
+  // LQ_SPMM_NT=0 turns it off, LQ_SPMM_NT=2 also streams T_{n-1} in launches
+  // without fused operators (2 % slower on the cache-resident DOS case).
+  // T_{n+1} itself is gathered by the next step and measured 4 % slower when
+  // stored the same way.
   FusedArgs fa = fused ? *fused : FusedArgs{};
   static const int nt = env_int("LQ_SPMM_NT", 1);
-  fa.nt = fa.nv > 0 ? nt : 0;
+  fa.nt = fa.nv > 0 ? (nt > 0) : (nt >= 2);
   if (fa.nv < 0 || fa.nv > 2 || (fa.nv > 0 && (fa.set < 0 || fa.set >= nsets || !fa.vidx))) {
     set_error("launch_spmm: bad fused operator arguments");
     return LQ_ERR_ARG;
@@ -828,13 +834,17 @@ int launch_spmm(hipStream_t st, const SellDev &A, double2 alpha, double2 beta, c
 int launch_spmm_batch(hipStream_t st, int kp, int unroll, const BatchJobDev *jobs, int njobs, int grid,
                       const int *tc, int slot) {
   if (njobs <= 0 || grid <= 0) return LQ_OK;
+  // T_{n-1} loaded non-temporally: the batch mix holds jobs whose vector
+  // blocks exceed the MALL (0.8 % faster over the mix); LQ_SPMM_NT=0 turns it off
+  static const int ntz = env_int("LQ_SPMM_NT", 1) >= 1 ? 1 : 0;
 #define LQ_BU(KPV, UV)                                                                                     \
   if (kp == KPV && unroll == UV) {                                                                       \
     if (256 / KPV <= 64 && spmm_pf())                                                                    \
       hipLaunchKernelGGL((k_spmm_batch<KPV, UV, (256 / KPV <= 64)>), dim3(grid), dim3(256), 0, st, jobs,  \
-                         njobs, tc, slot);                                                               \
+                         njobs, tc, slot, ntz);                                                          \
     else                                                                                                 \
-      hipLaunchKernelGGL((k_spmm_batch<KPV, UV>), dim3(grid), dim3(256), 0, st, jobs, njobs, tc, slot);  \
+      hipLaunchKernelGGL((k_spmm_batch<KPV, UV>), dim3(grid), dim3(256), 0, st, jobs, njobs, tc, slot,   \
+                         ntz);                                                                           \
     LQ_HIP_CHECK(hipGetLastError());                                                                     \
     return LQ_OK;                                                                                        \
   }

@@ -101,7 +101,10 @@ struct DictMat {
 // per entry from LDS instead of a 16-byte-per-lane load through the vector
 // memory path that also carries the gathers.  The grids of these kernels are
 // capped (grid-stride) so the one-off staging per block is amortised.
-constexpr int kDictGridCap = 2048;
+// 1024 blocks of 4 waves fill the 256 CUs once at 4 waves per SIMD (the fused
+// recursion step's occupancy): 2 % faster there than 2048, the same on the DOS
+// step; 512 or 1536 measured much slower.
+constexpr int kDictGridCap = 1024;
 
 // Deterministic sum of the per-block dot partials of one slot, p[b * width + w]
 // over b < grid for each w < width (width a power of two <= 256), by a

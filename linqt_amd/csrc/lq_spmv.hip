@@ -13,6 +13,9 @@
 // across the wave and the x gathers / y stores are 16-byte per lane.
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "lq_internal.h"
 
@@ -101,10 +104,12 @@ struct DictMat {
 // per entry from LDS instead of a 16-byte-per-lane load through the vector
 // memory path that also carries the gathers.  The grids of these kernels are
 // capped (grid-stride) so the one-off staging per block is amortised.
-// 1024 blocks of 4 waves fill the 256 CUs once at 4 waves per SIMD (the fused
-// recursion step's occupancy): 2 % faster there than 2048, the same on the DOS
-// step; 512 or 1536 measured much slower.
-constexpr int kDictGridCap = 1024;
+// By default a dictionary-form launch without dot fusion gets one full set of
+// resident blocks (CUs x blocks per CU at the kernel's occupancy, see
+// resident_blocks): on the fused recursion step that measured 2 % faster than
+// twice as many, and a grid that leaves the last round part-full (1.5 sets)
+// 13 % slower.  LQ_SPMM_GRIDCAP=n (n >= 256) fixes the cap instead.
+constexpr int kDictGridCap = 1024;  // used when the occupancy query fails
 
 // Deterministic sum of the per-block dot partials of one slot, p[b * width + w]
 // over b < grid for each w < width (width a power of two <= 256), by a
@@ -618,8 +623,8 @@ bool use_dict(const SellDev &A) { return A.ndict > 0; }
 }  // namespace
 
 void dict_schedule(SellDev &d) {
-  const int c = env_int("LQ_SPMM_GRIDCAP", kDictGridCap);
-  d.gridcap = c >= 256 ? c : kDictGridCap;
+  const int c = env_int("LQ_SPMM_GRIDCAP", 0);
+  d.gridcap = c >= 256 ? c : 0;  // 0: one set of resident blocks
   d.pf = env_int("LQ_SPMM_PF", 1) != 0;
 }
 
@@ -636,19 +641,52 @@ MatArgs mat_args(const SellDev &A) {
 // write streams it is bound by the HBM traffic, not by the round trips.
 bool spmm_pf() { return env_int("LQ_SPMM_PF", 1) != 0; }
 
+// Blocks of 256 threads of `kernel` resident on the current device at once
+// (CUs x blocks per CU at its occupancy), queried once per kernel and device.
+int resident_blocks(const void *kernel) {
+  static std::mutex mu;
+  static std::map<std::pair<const void *, int>, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return kDictGridCap;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_pair(kernel, dev);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int per_cu = 0, cus = 0;
+  int r = kDictGridCap;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && per_cu > 0 &&
+      cus > 0)
+    r = per_cu * cus;
+  else
+    (void)hipGetLastError();
+  cache.emplace(key, r);
+  return r;
+}
+
+// grid < 0: one set of resident blocks of the kernel that runs, at most groups.
 template <int KP, int DOT, int U, bool DICT, int NV>
 void launch_k_spmm(hipStream_t st, int grid, int n, const MatArgs &m, double2 alpha, double2 beta,
                    const SetArgs &sa, int groups, const double2 *phi, double2 *partial, int q, const FusedArgs &fa,
                    bool pf) {
+  auto go = [&](auto kernel) {
+    int gr = grid;
+    if (gr < 0) {
+      gr = resident_blocks((const void *)kernel);
+      if (gr > groups) gr = groups;
+    }
+    hipLaunchKernelGGL(kernel, dim3(gr), dim3(256), 0, st, n, m, alpha, beta, sa, groups, phi, partial, q, fa);
+  };
   if constexpr (DICT && NV == 0 && 256 / KP <= 64) {
     if (q == 0 && pf) {
-      hipLaunchKernelGGL((k_spmm<KP, DOT, U, DICT, NV, true>), dim3(grid), dim3(256), 0, st, n, m, alpha, beta, sa,
-                         groups, phi, partial, q, fa);
+      go(k_spmm<KP, DOT, U, DICT, NV, true>);
       return;
     }
   }
-  hipLaunchKernelGGL((k_spmm<KP, DOT, U, DICT, NV>), dim3(grid), dim3(256), 0, st, n, m, alpha, beta, sa, groups,
-                     phi, partial, q, fa);
+  go(k_spmm<KP, DOT, U, DICT, NV>);
 }
 
 template <int KP, int U, bool DICT>
@@ -660,12 +698,14 @@ int launch_spmm_kpu(hipStream_t st, const SellDev &A, double2 alpha, double2 bet
   if (dot_mode != kDotNone) {
     grid = groups < kDotGridCap ? groups : kDotGridCap;
   } else if (DICT) {
-    grid = groups < A.gridcap ? groups : A.gridcap;
+    grid = A.gridcap > 0 ? (groups < A.gridcap ? groups : A.gridcap) : -1;  // -1: resident set
   } else {
     q = xcd_q(groups);
     if (q > 0) grid = q * kXcds;
   }
-  if (grid_used) *grid_used = grid;
+  // the block count matters to callers of dot-fused launches (partial sums);
+  // launches that size themselves at launch time report 0
+  if (grid_used) *grid_used = grid > 0 ? grid : 0;
   const MatArgs m = mat_args(A);
   if (fa.nv > 0) {
     if constexpr (DICT) {
@@ -720,7 +760,7 @@ int launch_spmm_kp(hipStream_t st, const SellDev &A, double2 alpha, double2 beta
     // per SIMD); 8-entry chunks need 128 (4 waves) and measured faster on rows
     // of 9-12 entries despite the second chunk
     const int u = unroll_for(A.dwmax);
-    switch (fa.nv > 0 && u == 12 && !getenv("LQ_SPMM_UNROLL") ? 8 : u) {
+    switch (u) {
       case 4: LQ_KPU(4, true);
       case 8: LQ_KPU(8, true);
       case 12: LQ_KPU(12, true);

@@ -138,10 +138,11 @@ __device__ void reduce_slot(const double2 *__restrict__ p, int grid, int width, 
   if (t < width) out[t] = sh[t];
 }
 
-// Schedule knobs: LQ_SPMM_GRIDCAP overrides the dictionary-form grid cap and
-// LQ_SPMM_PF=0 turns the prefetched row groups off (both read when an
-// operator is uploaded), LQ_SPMM_UNROLL (4, 8, 12, 16; read once) sets the
-// gather chunk.  None of them changes the arithmetic of a row.
+// Schedule knobs: LQ_SPMM_GRIDCAP fixes the grid cap of dictionary-form
+// launches (default: one set of resident blocks) and LQ_SPMM_PF=0 turns the
+// prefetched schedule off (both read when an operator is uploaded),
+// LQ_SPMM_UNROLL (4, 8, 12, 16; read once) sets the gather chunk.  None of
+// them changes the arithmetic of a row.
 int env_int(const char *name, int dflt) {
   const char *e = getenv(name);
   return e && *e ? atoi(e) : dflt;
@@ -757,8 +758,9 @@ int launch_spmm_kp(hipStream_t st, const SellDev &A, double2 alpha, double2 beta
   return launch_spmm_kpu<KP, UV, DV>(st, A, alpha, beta, sa, dot_mode, phi, partial, grid_used, fa)
   if (use_dict(A)) {
     // with fused right-operator applies a 12-entry chunk needs 164 VGPRs (3 waves
-    // per SIMD); 8-entry chunks need 128 (4 waves) and measured faster on rows
-    // of 9-12 entries despite the second chunk
+    // per SIMD) and 8-entry chunks 128 (4 waves); with the grid sized to the
+    // resident blocks of each, one 12-entry chunk per row measured faster on
+    // rows of 9-12 entries than two 8-entry chunks
     const int u = unroll_for(A.dwmax);
     switch (u) {
       case 4: LQ_KPU(4, true);

@@ -355,8 +355,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards over independent states and reduces the moment table once; see the "
-              "design notes)", file=sys.stderr)
+              "(over several GPUs the path would split the random states between them and sum "
+              "the moment tables with one all-reduce; only the one-GPU share is built)", file=sys.stderr)
         sys.exit(2)
 
     import linqt_amd as lq

@@ -365,9 +365,31 @@ int build_dict_form(const lq_csr A, SellDev &d) {
   return LQ_OK;
 }
 
+// Sliced-ELL fill from a device CSR: one wavefront per 64-row slice, lane r
+// = row r of the slice; entry j of the row goes to sptr[s] + 64 j + r, the
+// slots past the row's length get its own column and value 0.
+__global__ __launch_bounds__(64) void k_sell_fill(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
+                                                  const double2 *__restrict__ val, const int *__restrict__ sptr,
+                                                  int *__restrict__ ocol, double2 *__restrict__ oval) {
+  const int s = blockIdx.x, r = threadIdx.x, i = s * kSlice + r;
+  const int base = sptr[s], w = (sptr[s + 1] - base) / kSlice;
+  const int q0 = i < n ? rowptr[i] : 0, len = i < n ? rowptr[i + 1] - q0 : 0;
+  for (int j = 0; j < w; ++j) {
+    const size_t p = (size_t)base + (size_t)j * kSlice + r;
+    if (j < len) {
+      ocol[p] = col[q0 + j];
+      oval[p] = val[q0 + j];
+    } else {
+      ocol[p] = i < n ? i : 0;
+      oval[p] = make_double2(0.0, 0.0);
+    }
+  }
+}
+
 // Lays the completed CSR out as sliced ELL (64-row slices, column-major
-// inside a slice) and uploads it.
-int upload_sell(lq_csr A) {
+// inside a slice) on the device: slice offsets from the host row pointers,
+// the entries by k_sell_fill from src (or from an upload of A's host CSR).
+int upload_sell(lq_csr A, const DevCsr *src = nullptr) {
   const int n = A->n;
   const int nslices = (n + kSlice - 1) / kSlice;
   std::vector<int> sptr(nslices + 1, 0);
@@ -387,26 +409,7 @@ int upload_sell(lq_csr A) {
     sptr[s + 1] = (int)next;
   }
   const size_t stored = (size_t)sptr[nslices];
-  std::vector<int> col(stored > 0 ? stored : 1);
-  std::vector<double2> val(stored > 0 ? stored : 1);
-  for (int s = 0; s < nslices; ++s) {
-    const int w = (sptr[s + 1] - sptr[s]) / kSlice;
-    for (int r = 0; r < kSlice; ++r) {
-      const int i = s * kSlice + r;
-      const int len = i < n ? A->rowptr[i + 1] - A->rowptr[i] : 0;
-      for (int j = 0; j < w; ++j) {
-        const size_t p = (size_t)sptr[s] + (size_t)j * kSlice + r;
-        if (j < len) {
-          const int q = A->rowptr[i] + j;
-          col[p] = A->col[q];
-          val[p] = d2(A->val[q].real(), A->val[q].imag());
-        } else {
-          col[p] = i < n ? i : 0;
-          val[p] = d2(0.0, 0.0);
-        }
-      }
-    }
-  }
+  const size_t nnz = A->col.size();
   free_dev(A->dev);
   SellDev d;
   d.n = n;
@@ -415,11 +418,37 @@ int upload_sell(lq_csr A) {
   d.wmax = wmax;
   dict_schedule(d);
   LQ_HIP_CHECK(hipMalloc(&d.sptr, sizeof(int) * sptr.size()));
-  LQ_HIP_CHECK(hipMalloc(&d.col, sizeof(int) * col.size()));
-  LQ_HIP_CHECK(hipMalloc(&d.val, sizeof(double2) * val.size()));
+  LQ_HIP_CHECK(hipMalloc(&d.col, sizeof(int) * (stored > 0 ? stored : 1)));
+  LQ_HIP_CHECK(hipMalloc(&d.val, sizeof(double2) * (stored > 0 ? stored : 1)));
   LQ_HIP_CHECK(hipMemcpy(d.sptr, sptr.data(), sizeof(int) * sptr.size(), hipMemcpyHostToDevice));
-  LQ_HIP_CHECK(hipMemcpy(d.col, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
-  LQ_HIP_CHECK(hipMemcpy(d.val, val.data(), sizeof(double2) * val.size(), hipMemcpyHostToDevice));
+  if (stored > 0) {
+    DevCsr up;
+    int *t_rowptr = nullptr, *t_col = nullptr;
+    double2 *t_val = nullptr;
+    if (!src) {  // upload A's host CSR for the fill
+      LQ_HIP_CHECK(hipMalloc(&t_rowptr, sizeof(int) * ((size_t)n + 1)));
+      LQ_HIP_CHECK(hipMalloc(&t_col, sizeof(int) * (nnz > 0 ? nnz : 1)));
+      LQ_HIP_CHECK(hipMalloc(&t_val, sizeof(double2) * (nnz > 0 ? nnz : 1)));
+      LQ_HIP_CHECK(hipMemcpy(t_rowptr, A->rowptr.data(), sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice));
+      if (nnz > 0) {
+        LQ_HIP_CHECK(hipMemcpy(t_col, A->col.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+        LQ_HIP_CHECK(hipMemcpy(t_val, A->val.data(), sizeof(double2) * nnz, hipMemcpyHostToDevice));
+      }
+      up.rowptr = t_rowptr;
+      up.col = t_col;
+      up.val = t_val;
+      src = &up;
+    }
+    hipLaunchKernelGGL(k_sell_fill, dim3(nslices), dim3(kSlice), 0, 0, n, src->rowptr, src->col, src->val, d.sptr,
+                       d.col, d.val);
+    const hipError_t e = hipGetLastError();
+    const hipError_t e2 = hipDeviceSynchronize();
+    if (t_rowptr) (void)hipFree(t_rowptr);
+    if (t_col) (void)hipFree(t_col);
+    if (t_val) (void)hipFree(t_val);
+    LQ_HIP_CHECK(e);
+    LQ_HIP_CHECK(e2);
+  }
   int rc = build_dict_form(A, d);
   A->dev = d;
   return rc;
@@ -499,12 +528,12 @@ int spmm(lq_ctx ctx, lq_csr A, double2 alpha, double2 beta, const VecSet *sets, 
 
 // A new operator around a host CSR already filled in (rowptr / col / val /
 // kind, rows sorted, columns ascending): uploads its device forms.
-int csr_adopt(lq_ctx ctx, lq_csr A, lq_csr *out) {
+int csr_adopt(lq_ctx ctx, lq_csr A, lq_csr *out, const DevCsr *src) {
   static std::atomic<unsigned long long> next_uid{1};
   A->ctx = ctx;
   A->uid = next_uid++;
   DeviceGuard g(ctx->device);
-  const int rc = upload_sell(A);
+  const int rc = upload_sell(A, src);
   if (rc != LQ_OK) {
     free_dev(A->dev);
     delete A;

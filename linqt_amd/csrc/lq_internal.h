@@ -249,8 +249,17 @@ struct lq_csr_s {
 };
 
 namespace lq {
+// The same CSR already in device memory (row pointers, columns, values), so
+// the device forms can be laid out without uploading it again.
+struct DevCsr {
+  const int *rowptr = nullptr;
+  const int *col = nullptr;
+  const double2 *val = nullptr;
+};
+
 // Takes ownership of A (host CSR filled in) and uploads it; on error A is freed.
-int csr_adopt(lq_ctx ctx, lq_csr A, lq_csr *out);
+// src: optional device copy of A's CSR (the host arrays stay A's).
+int csr_adopt(lq_ctx ctx, lq_csr A, lq_csr *out, const DevCsr *src = nullptr);
 
 // Makes a context's device current for the scope of a call.
 struct DeviceGuard {

@@ -316,6 +316,7 @@ extern "C" int lq_csr_from_hoppings(lq_ctx ctx, int nwann, int nhop, const int *
   A->fill = LQ_FILL_HERMITIAN_UPPER;
   A->rowptr.assign((size_t)n + 1, 0);
   int rc = LQ_OK;
+  DevCsr dsrc;  // the completed CSR on the device, for the sliced-ELL layout
   do {
     if (hipMemcpyAsync(A->rowptr.data() + 1, d_count, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, ctx->stream) !=
             hipSuccess ||
@@ -376,10 +377,13 @@ extern "C" int lq_csr_from_hoppings(lq_ctx ctx, int nwann, int nhop, const int *
       rc = LQ_ERR_HIP;
       break;
     }
+    dsrc.rowptr = d_rowptr;
+    dsrc.col = d_col;
+    dsrc.val = d_val;
   } while (false);
   if (rc != LQ_OK) {
     delete A;
     return rc;
   }
-  return csr_adopt(ctx, A, out);
+  return csr_adopt(ctx, A, out, dsrc.rowptr ? &dsrc : nullptr);
 }

@@ -277,15 +277,43 @@ void free_dev(SellDev &d) {
 // Dictionary form of the completed CSR (see SellDev), when the operator has
 // n < 2^24 rows and at most 256 distinct off-diagonal values (a zero value
 // counts when padding is needed).  Leaves d.ndict = 0 otherwise.
-int build_dict_form(const lq_csr A, SellDev &d) {
+// Dictionary-form fill from a device CSR: one wavefront per 64-row slice,
+// lane r = row r.  Off-diagonal entries in CSR order become packed words at
+// dsptr[s] + 64 j + r (padding: own row, table entry zero); the row's diagonal
+// entries are summed in CSR order into diag (when diag is not null).
+__global__ __launch_bounds__(64) void k_dict_fill(int n, const int *__restrict__ rowptr, const int *__restrict__ col,
+                                                  const double2 *__restrict__ val,
+                                                  const unsigned char *__restrict__ eid, const int *__restrict__ dsptr,
+                                                  unsigned zero, unsigned *__restrict__ wcol,
+                                                  double2 *__restrict__ diag) {
+  const int s = blockIdx.x, r = threadIdx.x, i = s * kSlice + r;
+  const int base = dsptr[s], w = (dsptr[s + 1] - base) / kSlice;
+  int j = 0;
+  if (i < n) {
+    double dx = 0.0, dy = 0.0;
+    for (int q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+      const int c = col[q];
+      if (c == i) {
+        dx += val[q].x;
+        dy += val[q].y;
+        continue;
+      }
+      wcol[(size_t)base + (size_t)j * kSlice + r] = (unsigned)c | ((unsigned)eid[q] << 24);
+      ++j;
+    }
+    if (diag) diag[i] = make_double2(dx, dy);
+  }
+  for (; j < w; ++j) wcol[(size_t)base + (size_t)j * kSlice + r] = (unsigned)(i < n ? i : 0) | (zero << 24);
+}
+
+int build_dict_form(const lq_csr A, SellDev &d, const DevCsr &csr) {
   const int n = A->n;
   if (n >= (1 << 24)) return LQ_OK;
   const char *env = getenv("LQ_SPMM_DICT");
   if (env && env[0] == '0') return LQ_OK;
   const int nslices = (n + kSlice - 1) / kSlice;
   DictIndex index;
-  std::vector<unsigned char> eid(A->col.size());  // table index of each off-diagonal entry
-  std::vector<double2> diag(n, d2(0.0, 0.0));
+  std::vector<unsigned char> eid(A->col.size() > 0 ? A->col.size() : 1);  // table index of each off-diagonal entry
   bool has_diag = false;
   std::vector<int> offd(n, 0);
   double lre = 0.0, lim = 0.0;  // last value looked up (runs of equal values are common)
@@ -293,8 +321,6 @@ int build_dict_form(const lq_csr A, SellDev &d) {
   for (int i = 0; i < n; ++i)
     for (int q = A->rowptr[i]; q < A->rowptr[i + 1]; ++q) {
       if (A->col[q] == i) {
-        diag[i].x += A->val[q].real();
-        diag[i].y += A->val[q].imag();
         has_diag = true;
         continue;
       }
@@ -332,35 +358,29 @@ int build_dict_form(const lq_csr A, SellDev &d) {
   }
   std::vector<double2> &dict = index.dict;
   const size_t stored = (size_t)sptr[nslices];
-  std::vector<unsigned> wcol(stored > 0 ? stored : 1, 0u);
-  for (int s = 0; s < nslices; ++s) {
-    const int w = (sptr[s + 1] - sptr[s]) / kSlice;
-    for (int r = 0; r < kSlice; ++r) {
-      const int i = s * kSlice + r;
-      int j = 0;
-      if (i < n)
-        for (int q = A->rowptr[i]; q < A->rowptr[i + 1]; ++q) {
-          if (A->col[q] == i) continue;
-          wcol[(size_t)sptr[s] + (size_t)j * kSlice + r] = (unsigned)A->col[q] | ((unsigned)eid[q] << 24);
-          ++j;
-        }
-      for (; j < w; ++j)
-        wcol[(size_t)sptr[s] + (size_t)j * kSlice + r] = (unsigned)(i < n ? i : 0) | ((unsigned)zero << 24);
-    }
-  }
   d.ndict = (int)std::max<size_t>(dict.size(), 1);
   if (dict.empty()) dict.push_back(d2(0.0, 0.0));
   d.dstored = (long long)stored;
   d.dwmax = wmax;
   LQ_HIP_CHECK(hipMalloc(&d.dsptr, sizeof(int) * sptr.size()));
-  LQ_HIP_CHECK(hipMalloc(&d.wcol, sizeof(unsigned) * wcol.size()));
+  LQ_HIP_CHECK(hipMalloc(&d.wcol, sizeof(unsigned) * (stored > 0 ? stored : 1)));
   LQ_HIP_CHECK(hipMalloc(&d.dict, sizeof(double2) * dict.size()));
   LQ_HIP_CHECK(hipMemcpy(d.dsptr, sptr.data(), sizeof(int) * sptr.size(), hipMemcpyHostToDevice));
-  LQ_HIP_CHECK(hipMemcpy(d.wcol, wcol.data(), sizeof(unsigned) * wcol.size(), hipMemcpyHostToDevice));
   LQ_HIP_CHECK(hipMemcpy(d.dict, dict.data(), sizeof(double2) * dict.size(), hipMemcpyHostToDevice));
-  if (has_diag) {
-    LQ_HIP_CHECK(hipMalloc(&d.diag, sizeof(double2) * n));
-    LQ_HIP_CHECK(hipMemcpy(d.diag, diag.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+  if (has_diag) LQ_HIP_CHECK(hipMalloc(&d.diag, sizeof(double2) * n));
+  if (n > 0) {
+    unsigned char *t_eid = nullptr;
+    LQ_HIP_CHECK(hipMalloc(&t_eid, eid.size()));
+    hipError_t e = hipMemcpy(t_eid, eid.data(), eid.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_dict_fill, dim3(nslices), dim3(kSlice), 0, 0, n, csr.rowptr, csr.col, csr.val, t_eid,
+                         d.dsptr, (unsigned)zero, d.wcol, d.diag);
+      e = hipGetLastError();
+    }
+    const hipError_t e2 = hipDeviceSynchronize();
+    (void)hipFree(t_eid);
+    LQ_HIP_CHECK(e);
+    LQ_HIP_CHECK(e2);
   }
   return LQ_OK;
 }
@@ -417,39 +437,50 @@ int upload_sell(lq_csr A, const DevCsr *src = nullptr) {
   d.stored = (long long)stored;
   d.wmax = wmax;
   dict_schedule(d);
-  LQ_HIP_CHECK(hipMalloc(&d.sptr, sizeof(int) * sptr.size()));
-  LQ_HIP_CHECK(hipMalloc(&d.col, sizeof(int) * (stored > 0 ? stored : 1)));
-  LQ_HIP_CHECK(hipMalloc(&d.val, sizeof(double2) * (stored > 0 ? stored : 1)));
-  LQ_HIP_CHECK(hipMemcpy(d.sptr, sptr.data(), sizeof(int) * sptr.size(), hipMemcpyHostToDevice));
-  if (stored > 0) {
-    DevCsr up;
-    int *t_rowptr = nullptr, *t_col = nullptr;
-    double2 *t_val = nullptr;
-    if (!src) {  // upload A's host CSR for the fill
-      LQ_HIP_CHECK(hipMalloc(&t_rowptr, sizeof(int) * ((size_t)n + 1)));
-      LQ_HIP_CHECK(hipMalloc(&t_col, sizeof(int) * (nnz > 0 ? nnz : 1)));
-      LQ_HIP_CHECK(hipMalloc(&t_val, sizeof(double2) * (nnz > 0 ? nnz : 1)));
-      LQ_HIP_CHECK(hipMemcpy(t_rowptr, A->rowptr.data(), sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice));
-      if (nnz > 0) {
-        LQ_HIP_CHECK(hipMemcpy(t_col, A->col.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
-        LQ_HIP_CHECK(hipMemcpy(t_val, A->val.data(), sizeof(double2) * nnz, hipMemcpyHostToDevice));
-      }
-      up.rowptr = t_rowptr;
-      up.col = t_col;
-      up.val = t_val;
-      src = &up;
-    }
-    hipLaunchKernelGGL(k_sell_fill, dim3(nslices), dim3(kSlice), 0, 0, n, src->rowptr, src->col, src->val, d.sptr,
-                       d.col, d.val);
-    const hipError_t e = hipGetLastError();
-    const hipError_t e2 = hipDeviceSynchronize();
+  // the CSR on the device for both fills (A's host arrays uploaded when no
+  // device copy was handed in); on an error everything allocated here is freed
+  DevCsr up;
+  int *t_rowptr = nullptr, *t_col = nullptr;
+  double2 *t_val = nullptr;
+  auto free_tmp = [&]() {
     if (t_rowptr) (void)hipFree(t_rowptr);
     if (t_col) (void)hipFree(t_col);
     if (t_val) (void)hipFree(t_val);
-    LQ_HIP_CHECK(e);
-    LQ_HIP_CHECK(e2);
+    t_rowptr = t_col = nullptr;
+    t_val = nullptr;
+  };
+  hipError_t e = hipSuccess;
+  auto ok = [&](hipError_t r) {
+    if (e == hipSuccess && r != hipSuccess) e = r;
+    return e == hipSuccess;
+  };
+  if (ok(hipMalloc(&d.sptr, sizeof(int) * sptr.size())) &&
+      ok(hipMalloc(&d.col, sizeof(int) * (stored > 0 ? stored : 1))) &&
+      ok(hipMalloc(&d.val, sizeof(double2) * (stored > 0 ? stored : 1))) &&
+      ok(hipMemcpy(d.sptr, sptr.data(), sizeof(int) * sptr.size(), hipMemcpyHostToDevice)) && !src) {
+    if (ok(hipMalloc(&t_rowptr, sizeof(int) * ((size_t)n + 1))) &&
+        ok(hipMalloc(&t_col, sizeof(int) * (nnz > 0 ? nnz : 1))) &&
+        ok(hipMalloc(&t_val, sizeof(double2) * (nnz > 0 ? nnz : 1))) &&
+        ok(hipMemcpy(t_rowptr, A->rowptr.data(), sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice)) &&
+        nnz > 0 && ok(hipMemcpy(t_col, A->col.data(), sizeof(int) * nnz, hipMemcpyHostToDevice)))
+      ok(hipMemcpy(t_val, A->val.data(), sizeof(double2) * nnz, hipMemcpyHostToDevice));
+    up.rowptr = t_rowptr;
+    up.col = t_col;
+    up.val = t_val;
+    src = &up;
   }
-  int rc = build_dict_form(A, d);
+  if (e == hipSuccess && stored > 0) {
+    hipLaunchKernelGGL(k_sell_fill, dim3(nslices), dim3(kSlice), 0, 0, n, src->rowptr, src->col, src->val, d.sptr,
+                       d.col, d.val);
+    if (ok(hipGetLastError())) ok(hipDeviceSynchronize());
+  }
+  if (e != hipSuccess) {
+    free_tmp();
+    free_dev(d);
+    LQ_HIP_CHECK(e);
+  }
+  int rc = build_dict_form(A, d, *src);
+  free_tmp();
   A->dev = d;
   return rc;
 }

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "lq_internal.h"
@@ -312,28 +313,80 @@ int build_dict_form(const lq_csr A, SellDev &d, const DevCsr &csr) {
   const char *env = getenv("LQ_SPMM_DICT");
   if (env && env[0] == '0') return LQ_OK;
   const int nslices = (n + kSlice - 1) / kSlice;
-  DictIndex index;
   std::vector<unsigned char> eid(A->col.size() > 0 ? A->col.size() : 1);  // table index of each off-diagonal entry
-  bool has_diag = false;
   std::vector<int> offd(n, 0);
-  double lre = 0.0, lim = 0.0;  // last value looked up (runs of equal values are common)
-  int lid = -1;
-  for (int i = 0; i < n; ++i)
-    for (int q = A->rowptr[i]; q < A->rowptr[i + 1]; ++q) {
-      if (A->col[q] == i) {
-        has_diag = true;
-        continue;
+  // Value indices in first-seen order, over contiguous row ranges on several
+  // threads: each range indexes its own values, then the range tables are
+  // merged in range order (which is the first-seen order over all entries)
+  // and the entries renumbered.
+  const int T = A->col.size() < ((size_t)1 << 20)
+                    ? 1
+                    : (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  struct Part {
+    DictIndex idx;
+    bool diag = false, full = false;
+    unsigned char map[256];
+  };
+  std::vector<Part> parts(T);
+  auto rows_of = [&](int t, int &i0, int &i1) {
+    i0 = (int)((long long)n * t / T);
+    i1 = (int)((long long)n * (t + 1) / T);
+  };
+  auto index_part = [&](int t) {
+    Part &P = parts[t];
+    int i0, i1;
+    rows_of(t, i0, i1);
+    double lre = 0.0, lim = 0.0;  // last value looked up (runs of equal values are common)
+    int lid = -1;
+    for (int i = i0; i < i1; ++i)
+      for (int q = A->rowptr[i]; q < A->rowptr[i + 1]; ++q) {
+        if (A->col[q] == i) {
+          P.diag = true;
+          continue;
+        }
+        ++offd[i];
+        const double re = A->val[q].real(), im = A->val[q].imag();
+        if (lid < 0 || !(re == lre && im == lim)) {
+          lid = P.idx.find_or_add(re, im);
+          if (lid < 0) {
+            P.full = true;
+            return;
+          }
+          lre = re;
+          lim = im;
+        }
+        eid[q] = (unsigned char)lid;
       }
-      ++offd[i];
-      const double re = A->val[q].real(), im = A->val[q].imag();
-      if (lid < 0 || !(re == lre && im == lim)) {
-        lid = index.find_or_add(re, im);
-        if (lid < 0) return LQ_OK;
-        lre = re;
-        lim = im;
-      }
-      eid[q] = (unsigned char)lid;
+  };
+  auto run = [&](auto &&f) {
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; ++t) th.emplace_back(f, t);
+    f(0);
+    for (auto &x : th) x.join();
+  };
+  run(index_part);
+  DictIndex index;
+  bool has_diag = false, renumber = false;
+  for (int t = 0; t < T; ++t) {
+    if (parts[t].full) return LQ_OK;
+    has_diag = has_diag || parts[t].diag;
+    const std::vector<double2> &loc = parts[t].idx.dict;
+    for (size_t k = 0; k < loc.size(); ++k) {
+      const int g = index.find_or_add(loc[k].x, loc[k].y);
+      if (g < 0) return LQ_OK;
+      parts[t].map[k] = (unsigned char)g;
+      renumber = renumber || g != (int)k;
     }
+  }
+  if (renumber)
+    run([&](int t) {
+      int i0, i1;
+      rows_of(t, i0, i1);
+      const unsigned char *m = parts[t].map;
+      for (int i = i0; i < i1; ++i)
+        for (int q = A->rowptr[i]; q < A->rowptr[i + 1]; ++q)
+          if (A->col[q] != i) eid[q] = m[eid[q]];
+    });
   std::vector<int> sptr(nslices + 1, 0);
   int wmax = 0;
   bool padded = false;

@@ -358,9 +358,15 @@ int build_dict_form(const lq_csr A, SellDev &d, const DevCsr &csr) {
         eid[q] = (unsigned char)lid;
       }
   };
-  auto run = [&](auto &&f) {
+  auto run = [&](auto &&f) {  // ranges whose thread cannot be started run here
     std::vector<std::thread> th;
-    for (int t = 1; t < T; ++t) th.emplace_back(f, t);
+    int t = 1;
+    try {
+      th.reserve(T);
+      for (; t < T; ++t) th.emplace_back(f, t);
+    } catch (...) {
+    }
+    for (int u = t; u < T; ++u) f(u);
     f(0);
     for (auto &x : th) x.join();
   };

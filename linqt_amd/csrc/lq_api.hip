@@ -32,6 +32,11 @@ void set_error(const char *fmt, ...) {
   g_last_error = buf;
 }
 
+int abi_fail(const char *what, int code) {
+  set_error("%s", what);
+  return code;
+}
+
 int ctx_buf(lq_ctx ctx, const char *name, size_t bytes, void **p) {
   auto &b = ctx->bufs[name];
   if (bytes == 0) bytes = 16;
@@ -894,7 +899,7 @@ const char *lq_last_error(void) { return g_last_error.c_str(); }
 
 int lq_abi_version(void) { return LQ_ABI_VERSION; }
 
-int lq_device_count(int *count) {
+int lq_device_count(int *count) try {
   if (!count) return LQ_ERR_ARG;
   hipError_t e = hipGetDeviceCount(count);
   if (e != hipSuccess) {
@@ -904,9 +909,9 @@ int lq_device_count(int *count) {
     return LQ_ERR_NODEV;
   }
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_ctx_create(int device, lq_ctx *out) {
+int lq_ctx_create(int device, lq_ctx *out) try {
   if (!out) {
     set_error("lq_ctx_create: null output");
     return LQ_ERR_ARG;
@@ -929,9 +934,9 @@ int lq_ctx_create(int device, lq_ctx *out) {
   }
   *out = c;
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_ctx_destroy(lq_ctx ctx) {
+int lq_ctx_destroy(lq_ctx ctx) try {
   if (!ctx) return LQ_OK;
   DeviceGuard g(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
@@ -945,38 +950,38 @@ int lq_ctx_destroy(lq_ctx ctx) {
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_ctx_synchronize(lq_ctx ctx) {
+int lq_ctx_synchronize(lq_ctx ctx) try {
   if (!ctx) return LQ_ERR_ARG;
   DeviceGuard g(ctx->device);
   LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return collect_timing(ctx);
-}
+} LQ_ABI_CATCH
 
-int lq_ctx_set_timing(lq_ctx ctx, int enabled) {
+int lq_ctx_set_timing(lq_ctx ctx, int enabled) try {
   if (!ctx) return LQ_ERR_ARG;
   ctx->timing = enabled != 0;
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_ctx_reset_timing(lq_ctx ctx) {
+int lq_ctx_reset_timing(lq_ctx ctx) try {
   if (!ctx) return LQ_ERR_ARG;
   DeviceGuard g(ctx->device);
   LQ_TRY(collect_timing(ctx));
   ctx->tsum = lq_timing{};
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_ctx_get_timing(lq_ctx ctx, lq_timing *out) {
+int lq_ctx_get_timing(lq_ctx ctx, lq_timing *out) try {
   if (!ctx || !out) return LQ_ERR_ARG;
   DeviceGuard g(ctx->device);
   LQ_TRY(collect_timing(ctx));
   *out = ctx->tsum;
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_malloc(lq_ctx ctx, size_t bytes, void **dptr) {
+int lq_malloc(lq_ctx ctx, size_t bytes, void **dptr) try {
   if (!ctx || !dptr) return LQ_ERR_ARG;
   DeviceGuard g(ctx->device);
   hipError_t e = hipMalloc(dptr, bytes ? bytes : 16);
@@ -987,44 +992,44 @@ int lq_malloc(lq_ctx ctx, size_t bytes, void **dptr) {
     return LQ_ERR_NOMEM;
   }
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_free(lq_ctx ctx, void *dptr) {
+int lq_free(lq_ctx ctx, void *dptr) try {
   if (!ctx) return LQ_ERR_ARG;
   if (!dptr) return LQ_OK;
   DeviceGuard g(ctx->device);
   LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   LQ_HIP_CHECK(hipFree(dptr));
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_memcpy_h2d(lq_ctx ctx, void *dst, const void *src, size_t bytes) {
+int lq_memcpy_h2d(lq_ctx ctx, void *dst, const void *src, size_t bytes) try {
   if (!ctx || (!dst && bytes) || (!src && bytes)) return LQ_ERR_ARG;
   DeviceGuard g(ctx->device);
   LQ_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
   LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_memcpy_d2h(lq_ctx ctx, void *dst, const void *src, size_t bytes) {
+int lq_memcpy_d2h(lq_ctx ctx, void *dst, const void *src, size_t bytes) try {
   if (!ctx || (!dst && bytes) || (!src && bytes)) return LQ_ERR_ARG;
   DeviceGuard g(ctx->device);
   LQ_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
   LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_memcpy_d2d(lq_ctx ctx, void *dst, const void *src, size_t bytes) {
+int lq_memcpy_d2d(lq_ctx ctx, void *dst, const void *src, size_t bytes) try {
   if (!ctx || (!dst && bytes) || (!src && bytes)) return LQ_ERR_ARG;
   DeviceGuard g(ctx->device);
   LQ_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
   LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
 // ---- operators ---------------------------------------------------------------
 int lq_csr_create(lq_ctx ctx, int n, int nnz, const int *rowptr, const int *col, const double *val, int fill,
-                  lq_csr *out) {
+                  lq_csr *out) try {
   if (!ctx || !out || n <= 0 || nnz < 0 || !rowptr || (nnz > 0 && (!col || !val))) {
     set_error("lq_csr_create: invalid arguments (n=%d nnz=%d)", n, nnz);
     return LQ_ERR_ARG;
@@ -1129,9 +1134,9 @@ int lq_csr_create(lq_ctx ctx, int n, int nnz, const int *rowptr, const int *col,
   A->fill = fill;
   set_entries(A, e);
   return csr_adopt(ctx, A, out);
-}
+} LQ_ABI_CATCH
 
-int lq_csr_export(lq_csr A, long long *nnz, int *rowptr, int *col, double *val) {
+int lq_csr_export(lq_csr A, long long *nnz, int *rowptr, int *col, double *val) try {
   LQ_TRY(check_op(A, "lq_csr_export"));
   if (!nnz || (rowptr && (!col || !val))) {
     set_error("lq_csr_export: null argument");
@@ -1154,9 +1159,9 @@ int lq_csr_export(lq_csr A, long long *nnz, int *rowptr, int *col, double *val) 
     rowptr[i + 1] = (int)q;
   }
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_csr_destroy(lq_csr A) {
+int lq_csr_destroy(lq_csr A) try {
   if (!A) return LQ_OK;
   DeviceGuard g(A->ctx->device);
   (void)hipStreamSynchronize(A->ctx->stream);
@@ -1164,12 +1169,12 @@ int lq_csr_destroy(lq_csr A) {
   free_fused(A->fused);
   delete A;
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
 // MKL's add C = a*U + b*I on the stored (upper) entries; the mirrored
 // entries stay the conjugates of the scaled upper ones, and a row without a
 // stored diagonal gains one (src/mkl_sparse_matrix.cpp:73-97).
-int lq_csr_rescale(lq_csr A, double a_re, double a_im, double b_re, double b_im) {
+int lq_csr_rescale(lq_csr A, double a_re, double a_im, double b_re, double b_im) try {
   LQ_TRY(check_op(A, "lq_csr_rescale"));
   const std::complex<double> a(a_re, a_im), b(b_re, b_im);
   std::vector<Entry> e = get_entries(A);
@@ -1194,17 +1199,17 @@ int lq_csr_rescale(lq_csr A, double a_re, double a_im, double b_re, double b_im)
   set_entries(A, e);
   A->version++;
   return upload_sell(A);
-}
+} LQ_ABI_CATCH
 
-int lq_csr_info(lq_csr A, int *n, long long *nnz, long long *nnz_stored) {
+int lq_csr_info(lq_csr A, int *n, long long *nnz, long long *nnz_stored) try {
   LQ_TRY(check_op(A, "lq_csr_info"));
   if (n) *n = A->n;
   if (nnz) *nnz = (long long)A->col.size();
   if (nnz_stored) *nnz_stored = A->dev.stored;
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_spmv(lq_csr A, double a_re, double a_im, const double *x, double b_re, double b_im, double *y, int k) {
+int lq_spmv(lq_csr A, double a_re, double a_im, const double *x, double b_re, double b_im, double *y, int k) try {
   LQ_TRY(check_op(A, "lq_spmv"));
   if (!x || !y || k <= 0 || k > 64 || x == y) {
     set_error("lq_spmv: need distinct x, y and 1 <= k <= 64");
@@ -1215,10 +1220,10 @@ int lq_spmv(lq_csr A, double a_re, double a_im, const double *x, double b_re, do
   const bool use_z = !(b_re == 0.0 && b_im == 0.0);
   VecSet s{(const double2 *)x, use_z ? (const double2 *)y : nullptr, (double2 *)y, k};
   return spmm(ctx, A, d2(a_re, a_im), d2(b_re, b_im), &s, 1);
-}
+} LQ_ABI_CATCH
 
 int lq_compute_mom_table_prec(lq_ctx ctx, const double *L, long long ldl, int ML, const double *R, long long ldr,
-                              int MR, long long K, int precision, double *sub, int ldsub) {
+                              int MR, long long K, int precision, double *sub, int ldsub) try {
   if (!ctx || !L || !R || !sub || ML <= 0 || MR <= 0 || K <= 0 || ldl < K || ldr < K || ldsub < MR) {
     set_error("lq_compute_mom_table: invalid arguments");
     return LQ_ERR_ARG;
@@ -1232,15 +1237,15 @@ int lq_compute_mom_table_prec(lq_ctx ctx, const double *L, long long ldl, int ML
                         ldsub, false, true));
   LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return collect_timing(ctx);
-}
+} LQ_ABI_CATCH
 
 int lq_compute_mom_table(lq_ctx ctx, const double *L, long long ldl, int ML, const double *R, long long ldr, int MR,
-                         long long K, double *sub, int ldsub) {
+                         long long K, double *sub, int ldsub) try {
   return lq_compute_mom_table_prec(ctx, L, ldl, ML, R, ldr, MR, K, LQ_PREC_FP64, sub, ldsub);
-}
+} LQ_ABI_CATCH
 
 // ---- 1D moments ----------------------------------------------------------------
-int lq_spectral_moments(lq_csr H, lq_csr OP, const double *phi, int R, int M, int algo, double *mu) {
+int lq_spectral_moments(lq_csr H, lq_csr OP, const double *phi, int R, int M, int algo, double *mu) try {
   LQ_TRY(check_op(H, "lq_spectral_moments"));
   if (!phi || R <= 0 || M <= 0 || !mu || (OP && (OP->n != H->n || OP->ctx != H->ctx))) {
     set_error("lq_spectral_moments: invalid arguments");
@@ -1365,7 +1370,7 @@ int lq_spectral_moments(lq_csr H, lq_csr OP, const double *phi, int R, int M, in
     mu[2 * m + 1] += acc[m].imag();
   }
   return collect_timing(ctx);
-}
+} LQ_ABI_CATCH
 
 // ---- 2D moments --------------------------------------------------------------------
 static int check_2d_ops(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, const char *what) {
@@ -1387,7 +1392,7 @@ static int check_2d_ops(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, const
 }
 
 int lq_correlation_moments(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, const double *phi, int R, int M0,
-                           int M1, int batch, int precision, double *mu) {
+                           int M1, int batch, int precision, double *mu) try {
   LQ_TRY(check_2d_ops(H, OPL, nopr, OPR, "lq_correlation_moments"));
   if (!phi || R <= 0 || M0 <= 0 || M1 <= 0 || !mu) {
     set_error("lq_correlation_moments: invalid arguments");
@@ -1424,11 +1429,11 @@ int lq_correlation_moments(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, co
     LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   }
   return collect_timing(ctx);
-}
+} LQ_ABI_CATCH
 
 // ---- Kubo FFT ------------------------------------------------------------------------
 int lq_kubo_fft(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, const double *phi, int R, int M, int num_sections,
-                int nump, int formula, int precision, double *final_data) {
+                int nump, int formula, int precision, double *final_data) try {
   LQ_TRY(check_2d_ops(H, OPL, nopr, OPR, "lq_kubo_fft"));
   const int n = H->n;
   if (!phi || R <= 0 || M <= 0 || nump <= 0 || !final_data || num_sections <= 0 || num_sections > n) {
@@ -1504,7 +1509,7 @@ int lq_kubo_fft(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, const double 
   for (auto m : owned) lq_csr_destroy(m);
   if (rc != LQ_OK) return rc;
   return collect_timing(ctx);
-}
+} LQ_ABI_CATCH
 
 // ---- time-dependent correlations ------------------------------------------------
 // chebyshev::TimeDependentCorrelations (src/chebyshev_solver.cpp:231-275) with
@@ -1512,7 +1517,7 @@ int lq_kubo_fft(lq_csr H, lq_csr OPL, int nopr, const lq_csr *OPR, const double 
 // the left and right vectors of a group evolve as the two sets of one SpMM per
 // Bessel term.
 int lq_time_correlations(lq_csr H, lq_csr OPL, lq_csr OPR, const double *phi, int R, int M, int nT, double dt,
-                         double W, double Ec, double *mu) {
+                         double W, double Ec, double *mu) try {
   LQ_TRY(check_op(H, "lq_time_correlations"));
   if (!phi || R <= 0 || M <= 0 || nT <= 0 || !mu || !(W > 0) || (OPL && (OPL->n != H->n || OPL->ctx != H->ctx)) ||
       (OPR && (OPR->n != H->n || OPR->ctx != H->ctx))) {
@@ -1629,9 +1634,9 @@ int lq_time_correlations(lq_csr H, lq_csr OPL, lq_csr OPR, const double *phi, in
     mu[2 * i + 1] += acc[i].imag();
   }
   return collect_timing(ctx);
-}
+} LQ_ABI_CATCH
 
-int lq_jackson_td(lq_ctx ctx, double *mu, int M, int nT, double W, double broad_meV, int *M_out) {
+int lq_jackson_td(lq_ctx ctx, double *mu, int M, int nT, double W, double broad_meV, int *M_out) try {
   if (!ctx || !mu || M <= 0 || nT <= 0 || !(broad_meV > 0) || !(W > 0) || !M_out) {
     set_error("lq_jackson_td: invalid arguments");
     return LQ_ERR_ARG;
@@ -1649,10 +1654,10 @@ int lq_jackson_td(lq_ctx ctx, double *mu, int M, int nT, double W, double broad_
   LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   *M_out = maxMom;
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
 int lq_timecorr_post(lq_ctx ctx, const double *mu, int M, int nT, int n, double W, double Ec, double dt, double EF,
-                     double *t, double *out) {
+                     double *t, double *out) try {
   if (!ctx || !mu || M <= 0 || nT <= 0 || !(W > 0) || !t || !out) {
     set_error("lq_timecorr_post: invalid arguments");
     return LQ_ERR_ARG;
@@ -1668,10 +1673,10 @@ int lq_timecorr_post(lq_ctx ctx, const double *mu, int M, int nT, int n, double 
   LQ_HIP_CHECK(hipMemcpyAsync(out, o + nT, nT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
 int lq_kubo_post(lq_ctx ctx, int formula, const double *final_data, int nump, int n, double W, double Ec, double *E,
-                 double *sigma, double *integrand) {
+                 double *sigma, double *integrand) try {
   if (!ctx || !final_data || nump <= 1 || n <= 0 || !E || !sigma ||
       (formula != LQ_KUBO_GREENWOOD && formula != LQ_KUBO_BASTIN)) {
     set_error("lq_kubo_post: invalid arguments");
@@ -1692,7 +1697,7 @@ int lq_kubo_post(lq_ctx ctx, int formula, const double *final_data, int nump, in
     LQ_HIP_CHECK(hipMemcpyAsync(integrand, di, nump * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
 // ---- Jackson + grid post-processors -----------------------------------------------------
 static int jackson_count(int M, double W, double broad) {
@@ -1701,7 +1706,7 @@ static int jackson_count(int M, double W, double broad) {
   return maxMom > M ? M : maxMom;
 }
 
-int lq_jackson_1d(lq_ctx ctx, double *mu, int M, double W, double broad, int *M_out) {
+int lq_jackson_1d(lq_ctx ctx, double *mu, int M, double W, double broad, int *M_out) try {
   if (!ctx || !mu || M <= 0 || !(broad > 0) || !(W > 0)) {
     set_error("lq_jackson_1d: invalid arguments");
     return LQ_ERR_ARG;
@@ -1716,9 +1721,9 @@ int lq_jackson_1d(lq_ctx ctx, double *mu, int M, double W, double broad, int *M_
   LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   if (M_out) *M_out = mm;
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_jackson_2d(lq_ctx ctx, double *mu, int M0, int M1, double W, double b0, double b1, int *M0_out, int *M1_out) {
+int lq_jackson_2d(lq_ctx ctx, double *mu, int M0, int M1, double W, double b0, double b1, int *M0_out, int *M1_out) try {
   if (!ctx || !mu || M0 <= 0 || M1 <= 0 || !(b0 > 0) || !(b1 > 0) || !(W > 0)) {
     set_error("lq_jackson_2d: invalid arguments");
     return LQ_ERR_ARG;
@@ -1735,10 +1740,10 @@ int lq_jackson_2d(lq_ctx ctx, double *mu, int M0, int M1, double W, double b0, d
   if (M0_out) *M0_out = mm0;
   if (M1_out) *M1_out = mm1;
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
 int lq_dos_fftgrid(lq_ctx ctx, const double *mu, int M, int n, double W, double Ec, int num_div, double *E,
-                   double *rho) {
+                   double *rho) try {
   if (!ctx || !mu || M <= 0 || num_div <= 0 || !E || !rho) {
     set_error("lq_dos_fftgrid: invalid arguments");
     return LQ_ERR_ARG;
@@ -1754,7 +1759,7 @@ int lq_dos_fftgrid(lq_ctx ctx, const double *mu, int M, int n, double W, double 
   LQ_HIP_CHECK(hipMemcpyAsync(rho, dout + num_div, num_div * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
 static int grid2d(lq_ctx ctx, int which, const double *mu, int M0, int M1, int n, double W, double Ec, int num_div,
                   double *E, double *out) {
@@ -1780,17 +1785,17 @@ static int grid2d(lq_ctx ctx, int which, const double *mu, int M0, int M1, int n
 }
 
 int lq_greenwood_fftgrid(lq_ctx ctx, const double *mu, int M0, int M1, int n, double W, double Ec, int num_div,
-                         double *E, double *sigma) {
+                         double *E, double *sigma) try {
   return grid2d(ctx, 0, mu, M0, M1, n, W, Ec, num_div, E, sigma);
-}
+} LQ_ABI_CATCH
 
 int lq_bastin_fftgrid(lq_ctx ctx, const double *mu, int M0, int M1, int n, double W, double Ec, int num_div, double *E,
-                      double *sigma) {
+                      double *sigma) try {
   return grid2d(ctx, 1, mu, M0, M1, n, W, Ec, num_div, E, sigma);
-}
+} LQ_ABI_CATCH
 
 int lq_linear_grid(lq_ctx ctx, int kind, const double *mu, int M0, int M1, int n, double W, double Ec, int num_div,
-                   double *E, double *out) {
+                   double *E, double *out) try {
   if (!ctx || !mu || kind < LQ_LIN_SPECTRAL || kind > LQ_LIN_BASTIN_SURF || M0 <= 0 ||
       (kind != LQ_LIN_SPECTRAL && M1 <= 0) || num_div <= 1 || !E || !out) {
     set_error("lq_linear_grid: invalid arguments");
@@ -1813,6 +1818,6 @@ int lq_linear_grid(lq_ctx ctx, int kind, const double *mu, int M0, int M1, int n
   LQ_HIP_CHECK(hipMemcpyAsync(out, dout + num_div, rows * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return collect_timing(ctx);
-}
+} LQ_ABI_CATCH
 
 }  // extern "C"

@@ -234,16 +234,16 @@ int run_chunk(lq_batch b) {
 
 extern "C" {
 
-int lq_batch_job_bytes(int n, int R, int M, int chunk, size_t *bytes) {
+int lq_batch_job_bytes(int n, int R, int M, int chunk, size_t *bytes) try {
   if (n <= 0 || R <= 0 || R > 64 || M <= 0 || chunk <= 0 || !bytes) {
     set_error("lq_batch_job_bytes: invalid arguments");
     return LQ_ERR_ARG;
   }
   *bytes = job_bytes(n, R, M, chunk);
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_batch_create(lq_ctx ctx, size_t hbm_budget, int chunk, lq_batch *out) {
+int lq_batch_create(lq_ctx ctx, size_t hbm_budget, int chunk, lq_batch *out) try {
   if (!ctx || !out || chunk < 0) {
     set_error("lq_batch_create: invalid arguments");
     return LQ_ERR_ARG;
@@ -277,9 +277,9 @@ int lq_batch_create(lq_ctx ctx, size_t hbm_budget, int chunk, lq_batch *out) {
   }
   *out = b;
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_batch_destroy(lq_batch b) {
+int lq_batch_destroy(lq_batch b) try {
   if (!b) return LQ_OK;
   DeviceGuard dg(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
@@ -292,9 +292,9 @@ int lq_batch_destroy(lq_batch b) {
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   delete b;
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_batch_submit(lq_batch b, lq_csr H, const double *phi, int R, int M, int *job) {
+int lq_batch_submit(lq_batch b, lq_csr H, const double *phi, int R, int M, int *job) try {
   if (!b || !H || !phi || R <= 0 || R > 64 || M <= 0 || !job || H->ctx != b->ctx) {
     set_error("lq_batch_submit: invalid arguments (1 <= R <= 64, H on the batch's context)");
     return LQ_ERR_ARG;
@@ -321,9 +321,9 @@ int lq_batch_submit(lq_batch b, lq_csr H, const double *phi, int R, int M, int *
   b->queue.push_back(j.id);
   b->jobs.emplace(j.id, std::move(j));
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_batch_run(lq_batch b, int max_chunks, int *running, int *queued) {
+int lq_batch_run(lq_batch b, int max_chunks, int *running, int *queued) try {
   if (!b) {
     set_error("lq_batch_run: null batch");
     return LQ_ERR_ARG;
@@ -340,9 +340,9 @@ int lq_batch_run(lq_batch b, int max_chunks, int *running, int *queued) {
   if (running) *running = run;
   if (queued) *queued = (int)b->queue.size();
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_batch_poll(lq_batch b, int job, int *state) {
+int lq_batch_poll(lq_batch b, int job, int *state) try {
   if (!b || !state) {
     set_error("lq_batch_poll: invalid arguments");
     return LQ_ERR_ARG;
@@ -354,9 +354,9 @@ int lq_batch_poll(lq_batch b, int job, int *state) {
   }
   *state = it->second.state;
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_batch_result(lq_batch b, int job, double *mu) {
+int lq_batch_result(lq_batch b, int job, double *mu) try {
   if (!b || !mu) {
     set_error("lq_batch_result: invalid arguments");
     return LQ_ERR_ARG;
@@ -373,9 +373,9 @@ int lq_batch_result(lq_batch b, int job, double *mu) {
   }
   b->jobs.erase(it);
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-int lq_batch_unroll(lq_batch b, int R, int *unroll) {
+int lq_batch_unroll(lq_batch b, int R, int *unroll) try {
   if (!b || !unroll || R <= 0 || R > 64) {
     set_error("lq_batch_unroll: invalid arguments");
     return LQ_ERR_ARG;
@@ -387,6 +387,6 @@ int lq_batch_unroll(lq_batch b, int R, int *unroll) {
   auto it = b->tune.find(pad2(R));
   *unroll = it == b->tune.end() ? 0 : it->second.best;
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
 }  // extern "C"

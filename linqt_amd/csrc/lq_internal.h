@@ -5,6 +5,8 @@
 
 #include <complex>
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -14,6 +16,21 @@
 namespace lq {
 
 void set_error(const char *fmt, ...);
+
+// Every C ABI entry point is a function-try-block ending in LQ_ABI_CATCH: no
+// C++ exception crosses the C boundary; it becomes an error code and the text
+// lq_last_error() returns.
+int abi_fail(const char *what, int code);
+#define LQ_ABI_CATCH                                                      \
+  catch (const std::bad_alloc &) {                                        \
+    return ::lq::abi_fail("host memory allocation failed", LQ_ERR_NOMEM); \
+  }                                                                       \
+  catch (const std::exception &e) {                                       \
+    return ::lq::abi_fail(e.what(), LQ_ERR_ARG);                          \
+  }                                                                       \
+  catch (...) {                                                           \
+    return ::lq::abi_fail("unexpected C++ exception", LQ_ERR_ARG);        \
+  }
 
 #define LQ_HIP_CHECK(call)                                                                    \
   do {                                                                                        \

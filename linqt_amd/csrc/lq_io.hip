@@ -243,7 +243,7 @@ const char *mom_kind_tag(int kind) {
 
 using namespace lq;
 
-extern "C" int lq_csr_file_read(const char *path, int *n, long long *nnz, int *rowptr, int *col, double *val) {
+extern "C" int lq_csr_file_read(const char *path, int *n, long long *nnz, int *rowptr, int *col, double *val) try {
   if (!path || !n || !nnz) {
     set_error("lq_csr_file_read: null argument");
     return LQ_ERR_ARG;
@@ -290,10 +290,10 @@ extern "C" int lq_csr_file_read(const char *path, int *n, long long *nnz, int *r
     return LQ_ERR_ARG;
   }
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
 extern "C" int lq_csr_file_write(const char *path, int format, int n, long long nnz, const int *rowptr,
-                                 const int *col, const double *val) {
+                                 const int *col, const double *val) try {
   if (!path || n <= 0 || nnz < 0 || !rowptr || (nnz > 0 && (!col || !val))) {
     set_error("lq_csr_file_write: invalid arguments");
     return LQ_ERR_ARG;
@@ -350,9 +350,9 @@ extern "C" int lq_csr_file_write(const char *path, int format, int n, long long 
     return LQ_ERR_ARG;
   }
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-extern "C" int lq_csr_file_load(lq_ctx ctx, const char *path, int fill, lq_csr *out) {
+extern "C" int lq_csr_file_load(lq_ctx ctx, const char *path, int fill, lq_csr *out) try {
   int n = 0;
   long long nnz = 0;
   LQ_TRY(lq_csr_file_read(path, &n, &nnz, nullptr, nullptr, nullptr));
@@ -360,9 +360,9 @@ extern "C" int lq_csr_file_load(lq_ctx ctx, const char *path, int fill, lq_csr *
   std::vector<double> val(nnz > 0 ? 2 * nnz : 2);
   LQ_TRY(lq_csr_file_read(path, &n, &nnz, rowptr.data(), col.data(), val.data()));
   return lq_csr_create(ctx, n, (int)nnz, rowptr.data(), col.data(), val.data(), fill, out);
-}
+} LQ_ABI_CATCH
 
-extern "C" int lq_mom_file_read(const char *path, lq_mom_header *h, double *mu) {
+extern "C" int lq_mom_file_read(const char *path, lq_mom_header *h, double *mu) try {
   if (!path || !h) {
     set_error("lq_mom_file_read: null argument");
     return LQ_ERR_ARG;
@@ -452,9 +452,9 @@ extern "C" int lq_mom_file_read(const char *path, lq_mom_header *h, double *mu) 
     return LQ_ERR_ARG;
   }
   return LQ_OK;
-}
+} LQ_ABI_CATCH
 
-extern "C" int lq_mom_file_write(const char *path, int format, const lq_mom_header *h, const double *mu) {
+extern "C" int lq_mom_file_write(const char *path, int format, const lq_mom_header *h, const double *mu) try {
   if (!path || !h || !mu || h->kind < 1 || h->kind > 3 || h->M0 <= 0 || h->M1 <= 0 || (h->kind == 1 && h->M1 != 1)) {
     set_error("lq_mom_file_write: invalid arguments");
     return LQ_ERR_ARG;
@@ -528,4 +528,4 @@ extern "C" int lq_mom_file_write(const char *path, int format, const lq_mom_head
     return LQ_ERR_ARG;
   }
   return LQ_OK;
-}
+} LQ_ABI_CATCH

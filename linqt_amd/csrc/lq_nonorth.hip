@@ -330,7 +330,7 @@ int check_square(lq_csr A, int n, const char *what) {
 using namespace lq;
 
 extern "C" int lq_overlap_solve(lq_csr S, const double *b, double *x, double tol, int maxit, int *iterations,
-                                double *error) {
+                                double *error) try {
   if (!S || !S->ctx || !b || !x || b == x) {
     set_error("lq_overlap_solve: need an operator and distinct device vectors b, x");
     return LQ_ERR_ARG;
@@ -342,10 +342,10 @@ extern "C" int lq_overlap_solve(lq_csr S, const double *b, double *x, double tol
   LQ_TRY(bc.init(ctx, S, buf));
   return bc.solve((const double2 *)b, (double2 *)x, tol > 0 ? tol : 1e-4, maxit > 0 ? maxit : 2000, iterations,
                   error);
-}
+} LQ_ABI_CATCH
 
 extern "C" int lq_spectral_moments_nonorth(lq_csr H, lq_csr OP, lq_csr S, const double *phi, int R, int M,
-                                           double tol, int maxit, double *mu, long long *iterations) {
+                                           double tol, int maxit, double *mu, long long *iterations) try {
   if (!H || !S || !phi || !mu || R <= 0 || M <= 0) {
     set_error("lq_spectral_moments_nonorth: invalid arguments");
     return LQ_ERR_ARG;
@@ -400,4 +400,4 @@ extern "C" int lq_spectral_moments_nonorth(lq_csr H, lq_csr OP, lq_csr S, const 
   LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   if (iterations) *iterations = bc.iterations;
   return LQ_OK;
-}
+} LQ_ABI_CATCH

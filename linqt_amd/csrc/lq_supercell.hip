@@ -224,7 +224,7 @@ inline int wrap(long long x, int b) { return (int)(((x % b) + b) % b); }
 using namespace lq;
 
 extern "C" int lq_csr_from_hoppings(lq_ctx ctx, int nwann, int nhop, const int *cell, const int *edge,
-                                    const double *val, const int *dims, lq_csr *out) {
+                                    const double *val, const int *dims, lq_csr *out) try {
   if (!ctx || !out || nwann <= 0 || nhop < 0 || !dims || (nhop > 0 && (!cell || !edge || !val))) {
     set_error("lq_csr_from_hoppings: invalid arguments");
     return LQ_ERR_ARG;
@@ -386,4 +386,4 @@ extern "C" int lq_csr_from_hoppings(lq_ctx ctx, int nwann, int nhop, const int *
     return rc;
   }
   return csr_adopt(ctx, A, out, dsrc.rowptr ? &dsrc : nullptr);
-}
+} LQ_ABI_CATCH

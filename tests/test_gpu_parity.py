@@ -394,6 +394,46 @@ def test_prefetched_row_groups_bitwise_equal(ctx, monkeypatch):
         assert np.array_equal(lq.spectral_moments(F, None, phi, 24), mu)
 
 
+def test_dictionary_values_indexed_across_row_ranges(ctx, monkeypatch):
+    """The dictionary form's value table is built over row ranges on several
+    threads and merged in first-seen order; an operator with more than 2^20
+    entries whose second half uses hopping values the first half never does
+    (so later ranges renumber their indices) gives the plain form's products
+    and the chain's own formula."""
+    n = 600_000
+    rows = np.arange(n)
+    t = np.where(rows < n // 2, -1.0 + 0.3j, 0.7 - 0.9j)  # hop (i, i+1) value
+    t[2 * n // 3:] = 0.25 + 0.5j
+    onsite = 0.1 * np.cos(rows)
+    hop = rows < n - 1
+    cnt = 1 + hop.astype(np.int64)
+    rowptr = np.concatenate([[0], np.cumsum(cnt)])
+    col = np.empty(rowptr[-1], np.int64)
+    val = np.empty(rowptr[-1], complex)
+    col[rowptr[:-1]] = rows
+    val[rowptr[:-1]] = onsite
+    h = rows[hop]
+    col[rowptr[h] + 1] = h + 1
+    val[rowptr[h] + 1] = t[h]
+    m = io.CSR(n, rowptr, col, val)
+    monkeypatch.setenv("LQ_SPMM_DICT", "0")
+    P = lq.SparseMatrix.from_csr(ctx, m)
+    monkeypatch.setenv("LQ_SPMM_DICT", "1")
+    D = lq.SparseMatrix.from_csr(ctx, m)
+    rng = np.random.default_rng(23)
+    x = rng.normal(size=(n, 2)) + 1j * rng.normal(size=(n, 2))
+    say("dictionary across row ranges", n)
+    yd, yp = D.Multiply(1.0, x), P.Multiply(1.0, x)
+    assert rel_inf(yd, yp) < 1e-14
+    for i in (0, 1, n // 2 - 1, n // 2, n // 2 + 1, 2 * n // 3 - 1, 2 * n // 3, 2 * n // 3 + 1, n - 1):
+        e = onsite[i] * x[i]
+        if i < n - 1:
+            e = e + t[i] * x[i + 1]
+        if i > 0:
+            e = e + np.conj(t[i - 1]) * x[i - 1]
+        assert np.allclose(yd[i], e, rtol=0, atol=1e-13), i
+
+
 def test_fused_right_operators_match_separate_applies(ctx, oracle, monkeypatch):
     """V_x, V_y (patterns inside H's) are applied inside the H step; results
     equal the separate applies, the oracle, and an operator outside H's

@@ -37,6 +37,43 @@ int abi_fail(const char *what, int code) {
   return code;
 }
 
+int ctx_pinned(lq_ctx ctx, size_t bytes, void **p) {
+  auto &b = ctx->pinned;
+  if (b.bytes < bytes) {
+    if (b.p) (void)hipHostFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    LQ_HIP_CHECK(hipHostMalloc(&b.p, bytes));
+    b.bytes = bytes;
+  }
+  *p = b.p;
+  return LQ_OK;
+}
+
+void host_copy(void *dst, const void *src, size_t bytes) {
+  const size_t chunk = (size_t)8 << 20;
+  const int T = (int)std::min<size_t>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())),
+                                      (bytes + chunk - 1) / chunk);
+  auto part = [&](int t) {
+    const size_t a = bytes * t / T, b = bytes * (t + 1) / T;
+    std::memcpy((char *)dst + a, (const char *)src + a, b - a);
+  };
+  if (T <= 1) {
+    if (bytes) std::memcpy(dst, src, bytes);
+    return;
+  }
+  std::vector<std::thread> th;  // parts whose thread cannot be started run here
+  int t = 1;
+  try {
+    th.reserve(T);
+    for (; t < T; ++t) th.emplace_back(part, t);
+  } catch (...) {
+  }
+  for (int u = t; u < T; ++u) part(u);
+  part(0);
+  for (auto &x : th) x.join();
+}
+
 int ctx_buf(lq_ctx ctx, const char *name, size_t bytes, void **p) {
   auto &b = ctx->bufs[name];
   if (bytes == 0) bytes = 16;
@@ -947,6 +984,7 @@ int lq_ctx_destroy(lq_ctx ctx) try {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+  if (ctx->pinned.p) (void)hipHostFree(ctx->pinned.p);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return LQ_OK;

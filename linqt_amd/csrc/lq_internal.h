@@ -246,6 +246,7 @@ struct lq_ctx_s {
     size_t bytes = 0;
   };
   std::unordered_map<std::string, Buf> bufs;
+  Buf pinned;  // grow-only pinned host staging (ctx_pinned)
 };
 
 // Host-side record of the completed operator (kept for rescale and for
@@ -295,6 +296,10 @@ bool is_device_ptr(const void *p, int device);
 int upload_states(lq_ctx ctx, const double *phi, int n, int r0, int g, double2 *dst);
 // Context helpers (lq_api.hip).
 int ctx_buf(lq_ctx ctx, const char *name, size_t bytes, void **p);
+// Grow-only pinned host buffer of the context (device-to-host staging).
+int ctx_pinned(lq_ctx ctx, size_t bytes, void **p);
+// dst[0, bytes) = src[0, bytes) on up to 16 host threads.
+void host_copy(void *dst, const void *src, size_t bytes);
 int time_begin(lq_ctx ctx, int cat, hipEvent_t *ev);
 int time_end(lq_ctx ctx, int cat, hipEvent_t ev, double amount, int launches = 1);
 // Bracket a run of SpMM launches with one event pair (bytes and launches summed).

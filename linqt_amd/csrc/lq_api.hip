@@ -1068,7 +1068,7 @@ int lq_memcpy_d2d(lq_ctx ctx, void *dst, const void *src, size_t bytes) try {
 // ---- operators ---------------------------------------------------------------
 int lq_csr_create(lq_ctx ctx, int n, int nnz, const int *rowptr, const int *col, const double *val, int fill,
                   lq_csr *out) try {
-  if (!ctx || !out || n <= 0 || nnz < 0 || !rowptr || (nnz > 0 && (!col || !val))) {
+  if (!out || n <= 0 || nnz < 0 || !rowptr || (nnz > 0 && (!col || !val))) {
     set_error("lq_csr_create: invalid arguments (n=%d nnz=%d)", n, nnz);
     return LQ_ERR_ARG;
   }
@@ -1079,6 +1079,21 @@ int lq_csr_create(lq_ctx ctx, int n, int nnz, const int *rowptr, const int *col,
   *out = nullptr;
   if (rowptr[0] != 0 || rowptr[n] != nnz) {
     set_error("lq_csr_create: rowptr[0]=%d rowptr[n]=%d, expected 0 and nnz=%d", rowptr[0], rowptr[n], nnz);
+    return LQ_ERR_ARG;
+  }
+  // Every row range inside [0, nnz] before col/val are touched: a row pointer
+  // that jumps past nnz and comes back would otherwise send the scans below
+  // beyond the arrays.
+  for (int i = 0; i < n; ++i)
+    if (rowptr[i + 1] < rowptr[i] || rowptr[i + 1] > nnz) {
+      set_error("lq_csr_create: rowptr decreases or exceeds nnz=%d at row %d (%d -> %d)", nnz, i, rowptr[i],
+                rowptr[i + 1]);
+      return LQ_ERR_ARG;
+    }
+  // The arrays are checked first, so a malformed operator is reported as such
+  // even without a device context.
+  if (!ctx) {
+    set_error("lq_csr_create: no context");
     return LQ_ERR_ARG;
   }
   // Rows with non-decreasing columns (every file the reference's writers

@@ -129,6 +129,9 @@ class Op:
         _check(lib().lqo_op_mv(self.h, a.real, a.imag, _d(x), b.real, b.imag, _d(y)), "mv")
         return y
 
+    def nnz(self) -> int:
+        return lib().lqo_op_nnz(self.h)
+
     def dense(self):
         d = np.zeros((self.n, self.n), dtype=np.complex128)
         _check(lib().lqo_op_dense(self.h, _d(d)), "dense")

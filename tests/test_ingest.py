@@ -82,6 +82,38 @@ def test_csr_reader_errors(tmp_path):
     assert r.n == 2 and r.val[0] == 1.5 and np.signbit(r.val[0].imag)
 
 
+def test_malformed_row_pointers_rejected_before_columns_are_read(tmp_path):
+    """A CSR file whose row pointers jump past nnz and come back (rowptr =
+    [0, 100, 5], nnz = 5) is read as written, and the operator constructor
+    rejects it with LQ_ERR_ARG before it scans any column: the check needs no
+    device context."""
+    import ctypes as C
+
+    from linqt_amd import _dp, _ip, _lib
+
+    p = tmp_path / "bad.CSR"
+    p.write_text("2 5\n" + " ".join(["1 0"] * 5) + "\n0 1 1 1 1\n0 100 5\n")
+    r = lq.read_csr_file(p)
+    assert list(r.rowptr) == [0, 100, 5] and r.nnz == 5
+    h = C.c_void_p()
+    for rowptr, nnz in (([0, 100, 5], 5), ([0, 3, 2, 5], 5), ([0, -1, 5], 5)):
+        rp = np.array(rowptr, np.int32)
+        n = rp.shape[0] - 1
+        col = np.zeros(nnz, np.int32)
+        val = np.zeros(nnz, np.complex128)
+        with pytest.raises(lq.LinqtError, match="rowptr decreases or exceeds") as e:
+            _lib.call("lq_csr_create", None, n, nnz, _ip(rp), _ip(col), _dp(val.view(np.float64)),
+                      lq.LQ_FILL_HERMITIAN_UPPER, C.byref(h))
+        assert e.value.code == _lib.LQ_ERR_ARG
+    # a well-formed operator passes the checks and stops at the missing context
+    rp = np.array([0, 1, 2], np.int32)
+    col = np.array([0, 1], np.int32)
+    val = np.ones(2, np.complex128)
+    with pytest.raises(lq.LinqtError, match="no context"):
+        _lib.call("lq_csr_create", None, 2, 2, _ip(rp), _ip(col), _dp(val.view(np.float64)),
+                  lq.LQ_FILL_HERMITIAN_UPPER, C.byref(h))
+
+
 def test_moment_files_reference_format_byte_for_byte(tmp_path):
     """The reference-written .chebmom2D files (15 digits, "-0" included) are
     reproduced exactly from the natively read values, which equal the numpy

@@ -1336,27 +1336,36 @@ int lq_spectral_moments(lq_csr H, lq_csr OP, const double *phi, int R, int M, in
       LQ_TRY(launch_copy(ctx->stream, A, phi_d, vec));
     }
     if (algo == LQ_ALGO_DIRECT) {
-      // slots m = 0..M-1 of width kp
-      LQ_TRY(ctx_buf(ctx, "part0", (size_t)M * grid * kp * sizeof(double2), (void **)&part0));
+      // Moment m's block partials go to slot m % kPartRing of a ring; each
+      // full ring (and the last, partial one) is reduced into sums0 before
+      // the ring is reused, so partials take kPartRing slots whatever M is.
+      // Every slot is reduced by itself in the same order as before.
+      LQ_TRY(ctx_buf(ctx, "part0", (size_t)kPartRing * grid * kp * sizeof(double2), (void **)&part0));
       LQ_TRY(ctx_buf(ctx, "sums0", (size_t)M * kp * sizeof(double2), (void **)&sums0));
       const size_t slot = (size_t)grid * kp;
-      LQ_TRY(launch_dot(ctx->stream, n, gg, phi_d, A, part0, nullptr));
+      auto flush = [&](int m_end) -> int {  // moments [m_end - filled, m_end) are in the ring
+        const int m0 = ((m_end - 1) / kPartRing) * kPartRing;
+        return launch_reduce_partials(ctx->stream, part0, m_end - m0, grid, kp, sums0 + (size_t)m0 * kp);
+      };
+      auto ring = [&](int m) { return part0 + (size_t)(m % kPartRing) * slot; };
+      LQ_TRY(launch_dot(ctx->stream, n, gg, phi_d, A, ring(0), nullptr));
       SpmvLoopGuard guard{ctx};
       LQ_TRY(spmv_loop_begin(ctx));
       if (M > 1) {
         VecSet s{A, nullptr, B, gg};
-        LQ_TRY(spmm(ctx, H, one, zero, &s, 1, kDotPhi, phi_d, part0 + slot));
+        LQ_TRY(spmm(ctx, H, one, zero, &s, 1, kDotPhi, phi_d, ring(1)));
       }
       for (int m = 1; m + 1 < M; ++m) {
+        if ((m + 1) % kPartRing == 0) LQ_TRY(flush(m + 1));
         // A <- 2 H~ B - A, then swap: after the step B holds T_{m+1}
         VecSet s{B, A, A, gg};
-        LQ_TRY(spmm(ctx, H, two, mone, &s, 1, kDotPhi, phi_d, part0 + (size_t)(m + 1) * slot));
+        LQ_TRY(spmm(ctx, H, two, mone, &s, 1, kDotPhi, phi_d, ring(m + 1)));
         std::swap(A, B);
       }
       LQ_TRY(spmv_loop_end(ctx));
       hipEvent_t ev;
       LQ_TRY(time_begin(ctx, kCatPost, &ev));
-      LQ_TRY(launch_reduce_partials(ctx->stream, part0, M, grid, kp, sums0));
+      LQ_TRY(flush(M));
       LQ_TRY(time_end(ctx, kCatPost, ev, 0));
       std::vector<double2> h((size_t)M * kp);
       LQ_HIP_CHECK(hipMemcpyAsync(h.data(), sums0, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
@@ -1371,7 +1380,8 @@ int lq_spectral_moments(lq_csr H, lq_csr OP, const double *phi, int R, int M, in
       const int nsteps = (M - 1) / 2;
       LQ_TRY(ctx_buf(ctx, "part0", (size_t)2 * grid * kp * sizeof(double2), (void **)&part0));
       LQ_TRY(ctx_buf(ctx, "sums0", (size_t)2 * kp * sizeof(double2), (void **)&sums0));
-      LQ_TRY(ctx_buf(ctx, "part1", (size_t)std::max(nsteps, 1) * grid * 2 * kp * sizeof(double2), (void **)&part1));
+      // step st's partials go to ring slot (st - 1) % kPartRing, reduced as for the direct path
+      LQ_TRY(ctx_buf(ctx, "part1", (size_t)kPartRing * grid * 2 * kp * sizeof(double2), (void **)&part1));
       LQ_TRY(ctx_buf(ctx, "sums1", (size_t)std::max(nsteps, 1) * 2 * kp * sizeof(double2), (void **)&sums1));
       const size_t slot0 = (size_t)grid * kp, slot1 = (size_t)grid * 2 * kp;
       LQ_TRY(launch_dot(ctx->stream, n, gg, A, A, part0, nullptr));
@@ -1382,17 +1392,22 @@ int lq_spectral_moments(lq_csr H, lq_csr OP, const double *phi, int R, int M, in
       }
       SpmvLoopGuard guard{ctx};
       LQ_TRY(spmv_loop_begin(ctx));
+      auto flush1 = [&](int s_end) -> int {  // steps' slots [s0, s_end) (zero-based) are in the ring
+        const int s0 = ((s_end - 1) / kPartRing) * kPartRing;
+        return launch_reduce_partials(ctx->stream, part1, s_end - s0, grid, 2 * kp, sums1 + (size_t)s0 * 2 * kp);
+      };
       for (int st = 1; st <= nsteps; ++st) {
+        if (st > 1 && (st - 1) % kPartRing == 0) LQ_TRY(flush1(st - 1));
         // x = T_st (B), z = y = T_st-1 (A): A <- T_st+1
         VecSet s{B, A, A, gg};
-        LQ_TRY(spmm(ctx, H, two, mone, &s, 1, kDotDoubling, nullptr, part1 + (size_t)(st - 1) * slot1));
+        LQ_TRY(spmm(ctx, H, two, mone, &s, 1, kDotDoubling, nullptr, part1 + (size_t)((st - 1) % kPartRing) * slot1));
         std::swap(A, B);
       }
       LQ_TRY(spmv_loop_end(ctx));
       hipEvent_t ev;
       LQ_TRY(time_begin(ctx, kCatPost, &ev));
       LQ_TRY(launch_reduce_partials(ctx->stream, part0, 2, grid, kp, sums0));
-      if (nsteps > 0) LQ_TRY(launch_reduce_partials(ctx->stream, part1, nsteps, grid, 2 * kp, sums1));
+      if (nsteps > 0) LQ_TRY(flush1(nsteps));
       LQ_TRY(time_end(ctx, kCatPost, ev, 0));
       std::vector<double2> h0((size_t)2 * kp), h1((size_t)std::max(nsteps, 1) * 2 * kp);
       LQ_HIP_CHECK(hipMemcpyAsync(h0.data(), sums0, sizeof(double2) * h0.size(), hipMemcpyDeviceToHost, ctx->stream));

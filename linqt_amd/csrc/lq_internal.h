@@ -51,6 +51,10 @@ int abi_fail(const char *what, int code);
 // wavefront's worth of rows when a lane owns a row.
 constexpr int kSlice = 64;
 
+// Slots of the ring that holds per-block dot partials of the 1D recursion
+// (one slot per moment or step); a full ring is reduced before it is reused.
+constexpr int kPartRing = 64;
+
 // Kernel-time categories recorded with HIP events.
 enum TimeCat { kCatSpmv = 0, kCatContract = 1, kCatPost = 2, kNumCats = 3 };
 

@@ -155,6 +155,24 @@ int lq_compute_mom_table_prec(lq_ctx ctx, const double *L, long long ldl, int ML
 int lq_spectral_moments(lq_csr H, lq_csr OP, const double *phi, int R, int M, int algo,
                         double *mu);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP stream/graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc): lq_spectral_moments with LQ_ALGO_DIRECT, same
+ * groups, kernels and summation order (the same moments, bit for bit), but
+ * every buffer beyond phi and mu lives in one fixed 64 MiB device array per
+ * GPU that the library holds until the process ends.  phi (R states,
+ * state-major) and mu (M complex, accumulated into) must be DEVICE memory of
+ * H's GPU.  The call only enqueues work on the context's stream (no
+ * allocation, no host synchronisation): mu is ready after
+ * lq_ctx_synchronize.  Sizes whose scratch exceeds 64 MiB -- about
+ * 16*M + 48*n*g + 256 KiB*kp bytes, g = min(R, 64), kp = g rounded up to a
+ * power of two -- return LQ_ERR_ARG with the required size in
+ * lq_last_error().  All contexts of one GPU share the array:
+ * calls that overlap on one GPU (two host threads, or two contexts whose
+ * streams run concurrently) race and give wrong results. */
+int lq_spectral_moments_noalloc(lq_csr H, lq_csr OP, const double *phi, int R, int M,
+                                double *mu);
+
 /* Replaces chebyshev::CorrelationExpansionMoments (src/chebyshev_solver.cpp:45-172)
  * for nopr right operators sharing one left operator: table o is
  * mu_o(m0,m1) = <phi| OPL T_m0 OPR_o T_m1 |phi>, scaled by 4/R with 1/2 on

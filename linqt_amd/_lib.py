@@ -113,6 +113,7 @@ _SIGS = {
     "lq_mom_file_read": (C.c_int, [C.c_char_p, C.c_void_p, _dp]),
     "lq_mom_file_write": (C.c_int, [C.c_char_p, C.c_int, C.c_void_p, _dp]),
     "lq_spectral_moments": (C.c_int, [_vp, _vp, _dp, C.c_int, C.c_int, C.c_int, _dp]),
+    "lq_spectral_moments_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "lq_correlation_moments": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(_vp), _dp, C.c_int, C.c_int, C.c_int,
                                          C.c_int, C.c_int, _dp]),
     "lq_kubo_fft": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(_vp), _dp, C.c_int, C.c_int, C.c_int, C.c_int,

@@ -961,6 +961,7 @@ int lq_ctx_create(int device, lq_ctx *out) try {
     return LQ_ERR_NODEV;
   }
   DeviceGuard g(device);
+  LQ_TRY(noalloc_init(device));
   lq_ctx c = new lq_ctx_s();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -298,6 +298,10 @@ bool is_device_ptr(const void *p, int device);
 // State-major phi (host, or device memory of the context's GPU) -> device
 // [n][g] interleaved block of states r0..r0+g-1.
 int upload_states(lq_ctx ctx, const double *phi, int n, int r0, int g, double2 *dst);
+// Looks up this device's address of the allocation-free entry point's scratch
+// array (lq_noalloc.hip); called once per device, with it current, when a
+// context is created.
+int noalloc_init(int device);
 // Context helpers (lq_api.hip).
 int ctx_buf(lq_ctx ctx, const char *name, size_t bytes, void **p);
 // Grow-only pinned host buffer of the context (device-to-host staging).

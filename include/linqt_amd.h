@@ -58,6 +58,13 @@ enum lq_algo1d {
 /* Contraction precision of the 2D table. */
 enum lq_precision {
   LQ_PREC_FP64 = 0,   /* FP64 MFMA */
+  /* Floating-point splits (the forms the split-precision literature uses):
+   * each FP64 operand is cut into lower-precision pieces whose products are
+   * summed on MFMA in FP32 accumulators, folded into FP64 every 256
+   * K-elements.  The FP32 accumulation limits them to ~1e-7 of the table;
+   * they exist to measure that, not for production tables. */
+  LQ_PREC_FP32X2 = 2, /* FP32 hi + lo, products hh + hl + lh on FP32 MFMA */
+  LQ_PREC_BF16X3 = 3, /* three bf16 pieces, products of levels i + j <= 2 on BF16 MFMA */
   /* Split precision: each operand is cut, per 4096-element group of a row,
    * into S int8 digits under a shared power-of-two exponent; digit products
    * of levels t + u <= S + 1 are summed exactly on INT8 MFMA (int32) and
@@ -139,6 +146,17 @@ int lq_csr_from_hoppings(lq_ctx ctx, int nwann, int nhop, const int *cell, const
 int lq_spmv(lq_csr A, double a_re, double a_im, const double *x, double b_re, double b_im,
             double *y, int k);
 
+/* No counterpart in the reference (MKL tunes its own kernels through
+ * mkl_sparse_set_mv_hint + mkl_sparse_optimize, src/mkl_sparse_matrix.cpp:48-51):
+ * times every schedule of A's recursion step at k vectors (mode 0: the step,
+ * 1: the step with <phi|T> fused) -- gather chunk 4/8/12/16, grid of
+ * resident blocks or 2048, prefetched row groups on/off -- and keeps the
+ * fastest in A.  Results do not change (the schedules do the same
+ * arithmetic).  Choices persist in LQ_TUNE_CACHE (default
+ * $HOME/.cache/linqt_amd/spmm_tune.txt) keyed by GPU and operator shape;
+ * *from_cache is 1 when read from there.  Outputs may be NULL. */
+int lq_csr_tune(lq_csr A, int k, int mode, int *unroll, int *gridcap, int *prefetch, int *from_cache);
+
 /* Replaces chebyshev::ComputeMomTable (src/chebyshev_solver.cpp:26-42):
  * sub[m0*ldsub + m1] = sum_k conj(L[m0*ldl + k]) * R[m1*ldr + k] for
  * m0 < ML, m1 < MR, k < K.  All DEVICE pointers; FP64 MFMA. */
@@ -216,6 +234,13 @@ int lq_overlap_solve(lq_csr S, const double *b, double *x, double tol, int maxit
  * OP == NULL is the identity.  iterations: total BiCGSTAB iterations (optional). */
 int lq_spectral_moments_nonorth(lq_csr H, lq_csr OP, lq_csr S, const double *phi, int R, int M, double tol,
                                 int maxit, double *mu, long long *iterations);
+
+/* No counterpart in the reference: measurement of the achievable HBM rate
+ * that the recursion kernels are compared against.  Streams `bytes` of a
+ * buffer of the context (kind 0: read, 1: copy between its halves, 2: write)
+ * with hand-written 16-byte non-temporal accesses, reps times; *ms is the
+ * mean time of one pass and *moved the bytes it reads plus writes. */
+int lq_hbm_probe(lq_ctx ctx, size_t bytes, int kind, int reps, double *ms, double *moved);
 
 /* ---- Jackson kernels and energy-grid post-processors --------------------- */
 /* Moments1D::ApplyJacksonKernel (src/chebyshev_moments1D.cpp:68-87);

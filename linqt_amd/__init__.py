@@ -28,7 +28,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (LQ_ALGO_DIRECT, LQ_ALGO_DOUBLING, LQ_FILL_GENERAL, LQ_FILL_HERMITIAN_UPPER,
-                   LQ_KUBO_BASTIN, LQ_KUBO_GREENWOOD, LQ_PREC_FP64, LQ_PREC_INT8X4, LQ_PREC_INT8X5,
+                   LQ_KUBO_BASTIN, LQ_KUBO_GREENWOOD, LQ_PREC_FP64, LQ_PREC_FP32X2, LQ_PREC_BF16X3, LQ_PREC_INT8X4, LQ_PREC_INT8X5,
                    LQ_PREC_INT8X6, LQ_PREC_INT8X7, LQ_FMT_TEXT, LQ_FMT_TEXT_REF, LQ_FMT_BINARY, LinqtError)
 
 __all__ = [
@@ -37,7 +37,7 @@ __all__ = [
     "timecorr_post", "spectral_moments", "spectral_moments_nonorth", "overlap_solve", "correlation_expansion_moments", "kubo_fft", "kubo_post",
     "jackson_1d", "jackson_2d", "dos_fftgrid", "greenwood_fftgrid", "bastin_fftgrid", "LinqtError",
     "LQ_FILL_HERMITIAN_UPPER", "LQ_FILL_GENERAL", "LQ_KUBO_GREENWOOD", "LQ_KUBO_BASTIN", "LQ_ALGO_DIRECT",
-    "LQ_ALGO_DOUBLING", "LQ_PREC_FP64", "LQ_PREC_INT8X4", "LQ_PREC_INT8X5", "LQ_PREC_INT8X6", "LQ_PREC_INT8X7",
+    "LQ_ALGO_DOUBLING", "LQ_PREC_FP64", "LQ_PREC_FP32X2", "LQ_PREC_BF16X3", "LQ_PREC_INT8X4", "LQ_PREC_INT8X5", "LQ_PREC_INT8X6", "LQ_PREC_INT8X7",
 ]
 
 
@@ -198,6 +198,13 @@ class SparseMatrix:
         a, b = complex(a), complex(b)
         _lib.call("lq_csr_rescale", self.h, a.real, a.imag, b.real, b.imag)
         return self
+
+    def tune(self, k: int, mode: int = 0) -> dict:
+        """Picks the fastest recursion-step schedule for k vectors (mode 1: with
+        the <phi|T> dot fused) and keeps it in this operator; see lq_csr_tune."""
+        u, g, p, c = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _lib.call("lq_csr_tune", self.h, k, mode, C.byref(u), C.byref(g), C.byref(p), C.byref(c))
+        return {"unroll": u.value, "gridcap": g.value, "prefetch": p.value, "from_cache": bool(c.value)}
 
     def info(self):
         n, nnz, st = C.c_int(), C.c_longlong(), C.c_longlong()

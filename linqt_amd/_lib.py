@@ -25,6 +25,8 @@ LQ_KUBO_BASTIN = 1
 LQ_ALGO_DIRECT = 0
 LQ_ALGO_DOUBLING = 1
 LQ_PREC_FP64 = 0
+LQ_PREC_FP32X2 = 2
+LQ_PREC_BF16X3 = 3
 LQ_LIN_SPECTRAL, LQ_LIN_GREENWOOD, LQ_LIN_BASTIN, LQ_LIN_BASTIN_I, LQ_LIN_BASTIN_II, LQ_LIN_BASTIN_SEA, \
     LQ_LIN_BASTIN_SURF = range(7)
 LQ_FMT_TEXT, LQ_FMT_TEXT_REF, LQ_FMT_BINARY = range(3)
@@ -114,6 +116,8 @@ _SIGS = {
     "lq_mom_file_write": (C.c_int, [C.c_char_p, C.c_int, C.c_void_p, _dp]),
     "lq_spectral_moments": (C.c_int, [_vp, _vp, _dp, C.c_int, C.c_int, C.c_int, _dp]),
     "lq_spectral_moments_noalloc": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, _vp]),
+    "lq_csr_tune": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, _ip, _ip]),
+    "lq_hbm_probe": (C.c_int, [_vp, C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "lq_correlation_moments": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(_vp), _dp, C.c_int, C.c_int, C.c_int,
                                          C.c_int, C.c_int, _dp]),
     "lq_kubo_fft": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(_vp), _dp, C.c_int, C.c_int, C.c_int, C.c_int,

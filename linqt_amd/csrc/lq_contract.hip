@@ -612,6 +612,29 @@ __global__ void k_contract_reduce(const double2 *__restrict__ work, int nsplit, 
   *o = make_double2(sx, sy);
 }
 
+// The same sum for a table whose columns hold nout operators' blocks of bc
+// columns side by side: column o*bc + s goes to out_o[m*ldo + s].
+__global__ void k_contract_reduce_cols(const double2 *__restrict__ work, int nsplit, int M0, int M1, int bc,
+                                       double2 *__restrict__ out0, double2 *__restrict__ out1, int ldo,
+                                       int accumulate) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)M0 * M1) return;
+  const int m = (int)(t / M1), nn = (int)(t % M1);
+  double sx = 0.0, sy = 0.0;
+  for (int c = 0; c < nsplit; ++c) {
+    const double2 w = work[((size_t)c * M0 + m) * M1 + nn];
+    sx += w.x;
+    sy += w.y;
+  }
+  const int o = nn / bc, sc = nn % bc;
+  double2 *o_ = (o == 0 ? out0 : out1) + (size_t)m * ldo + sc;
+  if (accumulate) {
+    sx += o_->x;
+    sy += o_->y;
+  }
+  *o_ = make_double2(sx, sy);
+}
+
 // Kernel choice for tables of at least 128 x 128: the three-multiplication
 // kernel (LQ_CONTRACT=4m selects the four-multiplication LDS kernel); smaller
 // tables, or LQ_CONTRACT=wave, use the register-only 64 x 64 kernel.
@@ -726,6 +749,19 @@ int launch_contract_reduce(hipStream_t st, const double2 *work, int nsplit, int 
   const long long total = (long long)M0 * M1;
   hipLaunchKernelGGL(k_contract_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, work, nsplit, M0,
                      M1, out, ldo, accumulate);
+  LQ_HIP_CHECK(hipGetLastError());
+  return LQ_OK;
+}
+
+int launch_contract_reduce_cols(hipStream_t st, const double2 *work, int nsplit, int M0, int M1, int bc,
+                                double2 *const *outs, int ldo, int accumulate) {
+  if (bc <= 0 || M1 % bc != 0 || M1 / bc > 2) {
+    set_error("launch_contract_reduce_cols: %d columns are not 1 or 2 blocks of %d", M1, bc);
+    return LQ_ERR_ARG;
+  }
+  const long long total = (long long)M0 * M1;
+  hipLaunchKernelGGL(k_contract_reduce_cols, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, work, nsplit,
+                     M0, M1, bc, outs[0], M1 / bc > 1 ? outs[1] : outs[0], ldo, accumulate);
   LQ_HIP_CHECK(hipGetLastError());
   return LQ_OK;
 }

@@ -87,6 +87,9 @@ struct SellDev {
   // uploaded (dict_schedule): grid cap and prefetched row groups
   int gridcap = 0;
   bool pf = true;
+  // gather chunk of every launch (4, 8, 12, 16; 0: the smallest covering the
+  // widest slice), set by the schedule tuner (lq_csr_tune)
+  int unroll = 0;
 };
 
 // Reads the schedule knobs LQ_SPMM_GRIDCAP (grid cap of dictionary-form
@@ -182,6 +185,10 @@ int launch_contract_f64(hipStream_t st, const double2 *A, long long lda, int M0,
                         long long ldb, int M1, long long K, double2 *work, int nsplit);
 int launch_contract_reduce(hipStream_t st, const double2 *work, int nsplit, int M0, int M1, double2 *out,
                            int ldo, int accumulate);
+// The same for M1 = nout*bc columns (nout <= 2 operators' blocks of bc
+// columns): column o*bc + s goes to outs[o][m*ldo + s].
+int launch_contract_reduce_cols(hipStream_t st, const double2 *work, int nsplit, int M0, int M1, int bc,
+                                double2 *const *outs, int ldo, int accumulate);
 
 // Split-precision contraction (lq_ozaki.hip).  launch_slice cuts rows x K
 // complex values (row stride ldx) into S int8 digit planes Q[row][t][re/im][Kp]
@@ -194,7 +201,14 @@ int launch_slice(hipStream_t st, const double2 *X, long long ldx, int rows, long
                  int *E);
 int launch_contract_i8(hipStream_t st, const signed char *QA, const int *EA, int M0, const signed char *QB,
                        const int *EB, int M1, long long K, int S, double2 *work, int nsplit);
-inline bool precision_ok(int p) { return p == 0 || (p >= 4 && p <= 7); }
+inline bool precision_ok(int p) { return p == 0 || p == 2 || p == 3 || (p >= 4 && p <= 7); }
+// Split precision with int8 digit planes (LQ_PREC_INT8X4..X7).
+inline bool precision_int8(int p) { return p >= 4 && p <= 7; }
+// Floating-point split forms (lq_split.hip): FP32 hi/lo or 3 x BF16 pieces,
+// FP32 MFMA accumulation folded into FP64 every 256 K-elements.
+int split_nsplit(int M0, int M1, long long K);
+int launch_contract_split(hipStream_t st, int precision, const double2 *A, long long lda, int M0, const double2 *B,
+                          long long ldb, int M1, long long K, double2 *work, int nsplit);
 
 // Post-processing (lq_post.hip).
 int launch_corr_epilogue(hipStream_t st, const double2 *raw, int ldr, const double2 *init, int M0,
@@ -225,6 +239,18 @@ int launch_linear_grid(hipStream_t st, int kind, const double2 *mu, int M0, int 
 
 }  // namespace lq
 
+// What a captured recursion graph was built for (lq_api.hip).
+struct GraphKey {
+  unsigned long long uid = 0;
+  int version = -1;
+  const void *a = nullptr, *b = nullptr, *phi = nullptr, *partial = nullptr;
+  int n = 0, k = 0, steps = 0;
+  bool operator==(const GraphKey &o) const {
+    return uid == o.uid && version == o.version && a == o.a && b == o.b && phi == o.phi && partial == o.partial &&
+           n == o.n && k == o.k && steps == o.steps;
+  }
+};
+
 // Context: one device, one stream, grow-only named scratch buffers.
 struct lq_ctx_s {
   int device = 0;
@@ -251,6 +277,15 @@ struct lq_ctx_s {
   };
   std::unordered_map<std::string, Buf> bufs;
   Buf pinned;  // grow-only pinned host staging (ctx_pinned)
+  // Second stream (default priority) for the moment contraction of the
+  // streamed 2D pipeline, and the timing-free events that order the two.
+  hipStream_t stream2 = nullptr;
+  std::vector<hipEvent_t> sync_events;
+  // the most recent captured recursion chunk
+  struct {
+    hipGraphExec_t exec = nullptr;
+    GraphKey key;
+  } graph;
 };
 
 // Host-side record of the completed operator (kept for rescale and for
@@ -310,6 +345,12 @@ int ctx_pinned(lq_ctx ctx, size_t bytes, void **p);
 void host_copy(void *dst, const void *src, size_t bytes);
 int time_begin(lq_ctx ctx, int cat, hipEvent_t *ev);
 int time_end(lq_ctx ctx, int cat, hipEvent_t ev, double amount, int launches = 1);
+// The same on another stream of the context (stream2).
+int time_begin_on(lq_ctx ctx, hipStream_t st, int cat, hipEvent_t *ev);
+int time_end_on(lq_ctx ctx, hipStream_t st, int cat, hipEvent_t ev, double amount, int launches = 1);
+// The context's second stream and n ordering events (created on first use).
+int ctx_stream2(lq_ctx ctx, hipStream_t *st);
+int ctx_sync_events(lq_ctx ctx, int n, hipEvent_t **ev);
 // Bracket a run of SpMM launches with one event pair (bytes and launches summed).
 int spmv_loop_begin(lq_ctx ctx);
 int spmv_loop_end(lq_ctx ctx);

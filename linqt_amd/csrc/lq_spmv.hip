@@ -745,9 +745,10 @@ int launch_spmm_kpu(hipStream_t st, const SellDev &A, double2 alpha, double2 bet
 
 // Unroll width: the smallest of 4/8/12/16 covering the widest slice (wider
 // operators loop over chunks of 16).
-int unroll_for(int wmax) {
+int unroll_for(int wmax, int tuned) {
   static const int forced = env_int("LQ_SPMM_UNROLL", 0);
   if (forced == 4 || forced == 8 || forced == 12 || forced == 16) return forced;
+  if (tuned == 4 || tuned == 8 || tuned == 12 || tuned == 16) return tuned;
   return wmax <= 4 ? 4 : wmax <= 8 ? 8 : wmax <= 12 ? 12 : 16;
 }
 
@@ -761,7 +762,7 @@ int launch_spmm_kp(hipStream_t st, const SellDev &A, double2 alpha, double2 beta
     // per SIMD) and 8-entry chunks 128 (4 waves); with the grid sized to the
     // resident blocks of each, one 12-entry chunk per row measured faster on
     // rows of 9-12 entries than two 8-entry chunks
-    const int u = unroll_for(A.dwmax);
+    const int u = unroll_for(A.dwmax, A.unroll);
     switch (u) {
       case 4: LQ_KPU(4, true);
       case 8: LQ_KPU(8, true);
@@ -769,7 +770,7 @@ int launch_spmm_kp(hipStream_t st, const SellDev &A, double2 alpha, double2 beta
       default: LQ_KPU(16, true);
     }
   }
-  switch (unroll_for(A.wmax)) {
+  switch (unroll_for(A.wmax, A.unroll)) {
     case 4: LQ_KPU(4, false);
     case 8: LQ_KPU(8, false);
     case 12: LQ_KPU(12, false);

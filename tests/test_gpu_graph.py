@@ -1,0 +1,76 @@
+"""Captured recursion chunks (HIP graphs) in the 1D solver.
+
+Full rings of 64 recursion steps are captured once and replayed, with the
+ring's reduction launched between replays.  The replayed kernels are the
+eager ones with the same arguments, so the moments must be equal bit for
+bit to the run with every step launched on its own (LQ_GRAPH=0, read once
+per process: each setting runs in a child process).  Every solve of the
+script replays the cached graph several times, with other work of the
+program (operator applies, a different solve that replaces the cached graph,
+host synchronisations) between the solves, and every result is compared.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+_SCRIPT = r"""
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import linqt_amd as lq
+from linqt_amd import lattices
+ctx = lq.Context(0)
+S = lattices.square_anderson(64, W=1.0, seed=11)
+H = lq.SparseMatrix.from_csr(ctx, S).Rescale(1 / 4.6, 0.0)
+rp = np.arange(S.n + 1, dtype=np.int32)
+OP = lq.SparseMatrix(ctx, S.n, rp, np.arange(S.n, dtype=np.int32), np.linspace(0.5, 1.5, S.n).astype(np.complex128))
+phi8 = lattices.random_phase_states(S.n, 8, seed=12)
+phi70 = lattices.random_phase_states(S.n, 70, seed=13)
+out = {}
+x = lattices.random_phase_states(S.n, 1, seed=14)[0]
+for rep in range(3):
+    out[f"c1_{rep}"] = lq.spectral_moments(H, None, phi8, 1024)
+    out[f"m200_{rep}"] = lq.spectral_moments(H, None, phi8, 200)
+    y = H.Multiply(1.0, x)          # eager work between solves
+    ctx.synchronize()
+    out[f"op_{rep}"] = lq.spectral_moments(H, OP, phi8, 300)
+    out[f"r70_{rep}"] = lq.spectral_moments(H, None, phi70, 260)
+    out[f"m100_{rep}"] = lq.spectral_moments(H, None, phi8, 100)
+    out[f"y_{rep}"] = y
+np.savez(sys.argv[2], **out)
+"""
+
+
+@pytest.fixture(scope="module")
+def runs(tmp_path_factory):
+    tmp = tmp_path_factory.mktemp("graph")
+    out = {}
+    for tag, val in (("graph", "1"), ("eager", "0")):
+        print("[case] LQ_GRAPH =", val, flush=True)
+        path = tmp / f"{tag}.npz"
+        env = {k: v for k, v in os.environ.items() if k != "LQ_GRAPH"}
+        env["LQ_GRAPH"] = val
+        r = subprocess.run([sys.executable, "-c", _SCRIPT, ROOT, str(path)], env=env, capture_output=True,
+                           text=True, timeout=600)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        out[tag] = dict(np.load(path))
+    return out
+
+
+def test_graph_replays_equal_eager_launches(runs):
+    g, e = runs["graph"], runs["eager"]
+    assert set(g) == set(e)
+    for key in sorted(g):
+        assert np.array_equal(g[key], e[key]), key
+    # every repetition of a solve gives the same moments (cached graph replayed)
+    for name in ("c1", "m200", "op", "r70", "m100"):
+        for rep in (1, 2):
+            assert np.array_equal(g[f"{name}_{rep}"], g[f"{name}_0"]), (name, rep)
+    assert abs(g["c1_0"][0] - 1.0) < 1e-12

@@ -132,8 +132,30 @@ def test_mom_table_fp64_precision_matches_plain(ctx):
     A, B = _operands(40, 30, 5000, seed=3, spread=False)
     got = _table_gpu(ctx, A, B, 0)
     assert rel_inf(got, A.conj() @ B.T) < 1e-12
-    with pytest.raises(lq.LinqtError):
-        _table_gpu(ctx, A, B, 3)
+    for bad in (1, 8, -1):
+        with pytest.raises(lq.LinqtError):
+            _table_gpu(ctx, A, B, bad)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [lq.LQ_PREC_FP32X2, lq.LQ_PREC_BF16X3])
+@pytest.mark.parametrize("ML,MR,K", [(70, 50, 3000), (130, 129, 300), (1, 3, 17), (65, 200, 8192)])
+def test_mom_table_float_splits(ctx, prec, ML, MR, K):
+    """FP32 hi/lo and 3 x BF16: every entry within the FP32-accumulation bound
+    (256-term FP32 sums, gamma_256 ~ 1.5e-5 of the summed magnitudes of the
+    three real products) of the exact table, and visibly not FP64-exact."""
+    say("float split table", prec, ML, MR, K)
+    A, B = _operands(ML, MR, K, seed=ML * 5 + MR + prec, spread=False)
+    got = _table_gpu(ctx, A, B, prec)
+    exact = A.conj() @ B.T
+    mag3 = (np.abs(A.real) + np.abs(A.imag)) @ (np.abs(B.real) + np.abs(B.imag)).T
+    err = np.abs(got - exact)
+    assert np.all(err <= 3e-5 * mag3 + 1e-300), float((err / mag3).max())
+    rel = rel_inf(got, exact)
+    print(f"precision {prec} {ML}x{MR}x{K}: rel err {rel:.2e}", flush=True)
+    assert rel < 1e-5
+    if K >= 300:
+        assert rel > 1e-13  # the FP32 accumulation shows
 
 
 @pytest.mark.gpu
@@ -152,6 +174,13 @@ def test_kubo_split_precision_vs_fp64(ctx):
         for o in range(2):
             sig = lq.kubo_post(ctx, 1, got[o], H.n, 9.2, 0.0)[1]
             errs[(S, o)] = rel_inf(sig, sig_ref[o])
+    for prec in (lq.LQ_PREC_FP32X2, lq.LQ_PREC_BF16X3):
+        say("kubo float split", prec)
+        got = lq.kubo_fft(G[0], G[1], [G[1], G[2]], phi, M, nump, lq.LQ_KUBO_BASTIN, precision=prec)
+        for o in range(2):
+            sig = lq.kubo_post(ctx, 1, got[o], H.n, 9.2, 0.0)[1]
+            errs[(prec, o)] = rel_inf(sig, sig_ref[o])
+            assert np.all(np.isfinite(sig)) and errs[(prec, o)] < 1.0
     print("sigma rel err", errs, flush=True)
     for o in range(2):
         assert errs[(7, o)] < 1e-8  # the accuracy target of the split path

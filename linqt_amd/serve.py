@@ -6,18 +6,23 @@ Requests arrive over time (a seeded Poisson stream in the benchmark):
   an energy grid -- these join the batched engine (``Batch``) at the next
   chunk boundary, so many of them advance together;
 * ``greenwood`` / ``bastin``: Kubo_solver_FFT::compute (Kubo_solver_FFT.cpp)
-  + post-processing -- run as whole solves between chunks, oldest first.
+  + post-processing -- handed to the backend's 2D workers (each its own
+  context, i.e. its own HIP stream and buffers), so their recursion and
+  contraction kernels run on the device beside the DOS batch's launches and
+  beside each other, and the loop keeps admitting arrivals and stepping the
+  batch while they run.
 
 Admission is by device footprint: a request starts only when the bytes it
 holds while running fit the server's HBM budget next to what is already
-running; otherwise it waits in arrival order (first fit, so a small request
-may pass a large one).  Every request's result is what the same request
-gives when served alone: the batched engine's moments are bitwise independent
-of the mix, and a 2D solve runs with the same inputs either way.
+running (and, for 2D requests, when a worker is free); otherwise it waits in
+arrival order (first fit, so a small request may pass a large one).  Every
+request's result is what the same request gives when served alone: the
+batched engine's moments are bitwise independent of the mix, and a 2D solve
+runs the same kernels on the same inputs on whichever context serves it.
 
 The scheduler (``Scheduler``) only sees a backend with ``footprint``,
-``start_1d``, ``step_1d``, ``run_2d`` and ``collect_1d``; ``GpuBackend``
-implements them on the C ABI.
+``start_1d``, ``step_1d``, ``collect_1d``, ``submit_2d``, ``poll_2d``,
+``wait_2d`` and ``slots_2d``; ``GpuBackend`` implements them on the C ABI.
 """
 from __future__ import annotations
 
@@ -58,16 +63,23 @@ def poisson_arrivals(n: int, rate: float, seed: int) -> np.ndarray:
 
 
 def request_mix(n: int, rate: float, seed: int, systems_1d: list[str], systems_2d: list[str],
-                frac_2d: float = 0.3, M_1d=(256, 512, 1024, 2048, 4096), M_2d=(128, 256), R=(1, 2, 4, 8)):
-    """n seeded requests: a fraction frac_2d Greenwood/Bastin, the rest DOS."""
+                frac_2d: float = 0.3, M_1d=(256, 512, 1024, 2048, 4096), M_2d=(128, 256), R=(1, 2, 4, 8),
+                sizes: dict | None = None, max_2d_work: float | None = None):
+    """n seeded requests: a fraction frac_2d Greenwood/Bastin, the rest DOS.
+    With sizes (system -> N) and max_2d_work, a 2D request's M is drawn from
+    the orders with M^2 N <= max_2d_work (the smallest order if none)."""
     rng = np.random.default_rng(seed + 1)
     t = poisson_arrivals(n, rate, seed)
     out = []
     for i in range(n):
         if systems_2d and rng.uniform() < frac_2d:
             kind = "greenwood" if rng.uniform() < 0.5 else "bastin"
-            out.append(Request(i, kind, str(rng.choice(systems_2d)), int(rng.choice(M_2d)),
-                               int(rng.choice(R[:2])), float(t[i]), seed=seed * 1000 + i))
+            system = str(rng.choice(systems_2d))
+            orders = list(M_2d)
+            if sizes is not None and max_2d_work is not None:
+                orders = [m for m in M_2d if m * m * sizes[system] <= max_2d_work] or [min(M_2d)]
+            out.append(Request(i, kind, system, int(rng.choice(orders)), int(rng.choice(R[:2])), float(t[i]),
+                               seed=seed * 1000 + i))
         else:
             out.append(Request(i, "dos", str(rng.choice(systems_1d)), int(rng.choice(M_1d)),
                                int(rng.choice(R)), float(t[i]), seed=seed * 1000 + i))
@@ -75,7 +87,12 @@ def request_mix(n: int, rate: float, seed: int, systems_1d: list[str], systems_2
 
 
 class Scheduler:
-    """Arrival-ordered, footprint-admitted continuous batching."""
+    """Arrival-ordered, footprint-admitted continuous batching.
+
+    One loop: admit arrived requests (DOS into the batch, 2D to a free
+    worker), advance the batch by one chunk, collect finished 2D solves.  The
+    loop never waits on a 2D solve; with nothing to step it sleeps until the
+    next arrival or until a 2D solve finishes."""
 
     def __init__(self, backend, hbm_budget: int, clock: Callable[[], float] | None = None,
                  sleep: Callable[[float], None] | None = None):
@@ -93,39 +110,51 @@ class Scheduler:
                 raise ValueError(f"request {r.rid} needs {self.b.footprint(r)} bytes, above the budget")
         waiting: list[Request] = []
         running_1d: dict[int, Request] = {}
+        running_2d: dict[int, Request] = {}
         t0 = self.clock()
         now = lambda: self.clock() - t0
-        while pending or waiting or running_1d:
+
+        def finish(r):
+            r.finish = now()
+            self.used -= self.b.footprint(r)
+            self.log.append((r.finish, "finish", r.rid))
+
+        while pending or waiting or running_1d or running_2d:
             while pending and pending[0].arrival <= now():
                 waiting.append(pending.pop(0))
-            # first-fit admission of 1D requests (they share the batched engine)
+            # first-fit admission in arrival order: DOS into the batch, 2D to a free worker
             for r in list(waiting):
-                if r.kind == "dos" and self.used + self.b.footprint(r) <= self.budget:
-                    waiting.remove(r)
-                    self.used += self.b.footprint(r)
-                    r.start = now()
+                fp = self.b.footprint(r)
+                if self.used + fp > self.budget:
+                    continue
+                if r.kind != "dos" and len(running_2d) >= self.b.slots_2d:
+                    continue
+                waiting.remove(r)
+                self.used += fp
+                r.start = now()
+                self.log.append((r.start, "start", r.rid))
+                if r.kind == "dos":
                     self.b.start_1d(r)
                     running_1d[r.rid] = r
-                    self.log.append((r.start, "start", r.rid))
-            # the oldest 2D request that fits runs whole, between chunks
-            for r in list(waiting):
-                if r.kind != "dos" and self.used + self.b.footprint(r) <= self.budget:
-                    waiting.remove(r)
-                    r.start = now()
-                    self.log.append((r.start, "start", r.rid))
-                    r.result = self.b.run_2d(r)
-                    r.finish = now()
-                    self.log.append((r.finish, "finish", r.rid))
-                    break
+                else:
+                    running_2d[r.rid] = r
+                    self.b.submit_2d(r)
             if running_1d:
                 for rid in self.b.step_1d():
                     r = running_1d.pop(rid)
                     r.result = self.b.collect_1d(r)
-                    r.finish = now()
-                    self.used -= self.b.footprint(r)
-                    self.log.append((r.finish, "finish", r.rid))
-            elif not waiting and pending:
-                self.sleep(max(0.0, pending[0].arrival - now()))
+                    finish(r)
+            for rid, res in self.b.poll_2d():
+                r = running_2d.pop(rid)
+                r.result = res
+                finish(r)
+            if not running_1d:
+                # nothing to step: sleep until the next arrival or a 2D solve ends
+                dt = max(0.0, pending[0].arrival - now()) if pending else None
+                if running_2d:
+                    self.b.wait_2d(dt if dt is not None else 0.05)
+                elif dt is not None and not waiting:
+                    self.sleep(dt)
         return requests
 
 
@@ -146,16 +175,36 @@ def summary(requests: list[Request]) -> dict:
 
 
 class GpuBackend:
-    """Runs requests on one context.  ``systems`` maps a name to
-    (H, V_x, V_y, n, W, Ec) with H already rescaled (V may be None for DOS-only)."""
+    """Runs requests on the GPU.  ``systems`` maps a name to (H, V_x, V_y, n, W,
+    Ec) with H already rescaled on context ``ctx`` (V may be None for DOS-only
+    systems); DOS requests run in a ``Batch`` on ``ctx``.  ``workers_2d`` is a
+    list of (context, systems) pairs -- the same systems uploaded on that
+    context -- and each runs one Greenwood/Bastin solve at a time on its own
+    host thread; with none, 2D solves run on ``ctx`` when submitted."""
 
-    def __init__(self, lq, ctx, systems: dict, chunk: int = 32, hbm_budget: int = 0):
+    def __init__(self, lq, ctx, systems: dict, chunk: int = 32, hbm_budget: int = 0, workers_2d=None):
+        import queue
+        import threading
+
         self.lq = lq
         self.ctx = ctx
         self.systems = systems
         self.batch = lq.Batch(ctx, hbm_budget=hbm_budget, chunk=chunk)
         self.chunk = chunk
         self.jobs: dict[int, int] = {}
+        self.workers = list(workers_2d or [])
+        self.slots_2d = max(1, len(self.workers))
+        self._done_2d: list[tuple[int, object]] = []
+        self._cv = threading.Condition()
+        self._queues = []
+        self._threads = []
+        self._busy = [0] * len(self.workers)
+        for i, (wctx, wsys) in enumerate(self.workers):
+            q = queue.Queue()
+            t = threading.Thread(target=self._worker, args=(i, wctx, wsys, q), daemon=True)
+            t.start()
+            self._queues.append(q)
+            self._threads.append(t)
 
     def states(self, r: Request):
         from .lattices import random_phase_states
@@ -166,8 +215,11 @@ class GpuBackend:
         n = self.systems[r.system][3]
         if r.kind == "dos":
             return self.lq.Batch.job_bytes(n, r.R, r.M, self.chunk)
-        # stored L and R vectors of both sides (one group of R states) + work vectors
-        return (2 * r.M + 4) * n * r.R * 16
+        # the 2D pipeline's buffers for one group of R states: the left vectors,
+        # two blocks of right products, work vectors (the streamed pipeline
+        # bounds this when M N is large; the resident one stores M right
+        # vectors per operator)
+        return min((3 * r.M + 4), (r.M + 4 * 128 + 5)) * n * r.R * 16 if r.M > 512 else (2 * r.M + 4) * n * r.R * 16
 
     def start_1d(self, r: Request):
         self.jobs[self.batch.submit(self.systems[r.system][0], self.states(r), r.M)] = r.rid
@@ -185,13 +237,57 @@ class GpuBackend:
         muj = lq.jackson_1d(self.ctx, mu, W, 1e-3)
         return mu, lq.dos_fftgrid(self.ctx, muj, n, W, Ec, r.nump or 2048)
 
-    def run_2d(self, r: Request):
+    def _solve_2d(self, ctx, systems, r: Request):
         lq = self.lq
-        H, Vx, Vy, n, W, Ec = self.systems[r.system]
+        H, Vx, Vy, n, W, Ec = systems[r.system]
         formula = lq.LQ_KUBO_GREENWOOD if r.kind == "greenwood" else lq.LQ_KUBO_BASTIN
         nump = r.nump or 2 * r.M
         fd = lq.kubo_fft(H, Vx, Vx, self.states(r), r.M, nump, formula)
-        return fd, lq.kubo_post(self.ctx, formula, fd, n, W, Ec)
+        return fd, lq.kubo_post(ctx, formula, fd, n, W, Ec)
+
+    def _worker(self, i, ctx, systems, q):
+        while True:
+            r = q.get()
+            if r is None:
+                return
+            try:
+                res = self._solve_2d(ctx, systems, r)
+            except Exception as e:  # reported by poll_2d's caller through the result
+                res = e
+            with self._cv:
+                self._busy[i] -= 1
+                self._done_2d.append((r.rid, res))
+                self._cv.notify_all()
+
+    def submit_2d(self, r: Request):
+        if not self.workers:
+            res = self._solve_2d(self.ctx, self.systems, r)
+            with self._cv:
+                self._done_2d.append((r.rid, res))
+            return
+        with self._cv:
+            i = min(range(len(self.workers)), key=lambda w: self._busy[w])
+            self._busy[i] += 1
+        self._queues[i].put(r)
+
+    def poll_2d(self) -> list[tuple[int, object]]:
+        with self._cv:
+            out, self._done_2d = self._done_2d, []
+        for rid, res in out:
+            if isinstance(res, Exception):
+                raise res
+        return out
+
+    def wait_2d(self, timeout: float):
+        with self._cv:
+            if not self._done_2d:
+                self._cv.wait(timeout)
+
+    def run_2d(self, r: Request):
+        """A 2D request solved synchronously on the first 2D context."""
+        if self.workers:
+            return self._solve_2d(self.workers[0][0], self.workers[0][1], r)
+        return self._solve_2d(self.ctx, self.systems, r)
 
     def alone(self, r: Request):
         """The same request served by itself (fresh batch for DOS)."""
@@ -208,4 +304,8 @@ class GpuBackend:
         return mu, lq.dos_fftgrid(self.ctx, muj, n, W, Ec, r.nump or 2048)
 
     def close(self):
+        for q in self._queues:
+            q.put(None)
+        for t in self._threads:
+            t.join()
         self.batch.close()

@@ -39,6 +39,7 @@ a bounded sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -49,7 +50,42 @@ import numpy as np
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
 INT8_MFMA_PEAK_TOPS = 5000.0
-PRECISIONS = {"fp64": 0, "int8x4": 4, "int8x5": 5, "int8x6": 6, "int8x7": 7}
+FP32_MFMA_PEAK_TFS = 157.3
+BF16_MFMA_PEAK_TFS = 2500.0
+PRECISIONS = {"fp64": 0, "fp32x2": 2, "bf16x3": 3, "int8x4": 4, "int8x5": 5, "int8x6": 6, "int8x7": 7}
+
+
+def kernel_source_sha16():
+    """Hash of the HIP sources the kernels are built from (the key of the
+    committed PMC traffic summary)."""
+    import glob
+    import hashlib
+
+    h = hashlib.sha256()
+    root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "linqt_amd", "csrc")
+    for path in sorted(glob.glob(os.path.join(root, "*.hip")) + glob.glob(os.path.join(root, "*.h"))):
+        h.update(os.path.basename(path).encode())
+        with open(path, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def host_cpu():
+    """Model name and logical CPUs this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count()
+    return {"model": model, "logical_cpus_available": n}
 
 
 def _pct(xs, q):
@@ -81,22 +117,25 @@ def cpu_baseline_bastin(host_ops, M, R, nump, phi):
     y = np.zeros_like(x)
     O.omp_cheb_step(Ho, x, y)  # builds the completed CSR outside the timing
     O.omp_mv(Vo, x, y)
-    reps = 5
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        O.omp_cheb_step(Ho, x, y)
-    tH = (time.perf_counter() - t0) / reps
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        O.omp_mv(Vo, x, y)
-    tV = (time.perf_counter() - t0) / reps
+    reps, samples = 5, 7
+
+    def median_time(f, k):
+        ts = []
+        for _ in range(samples):
+            t0 = time.perf_counter()
+            for _ in range(k):
+                f()
+            ts.append((time.perf_counter() - t0) / k)
+        return float(np.median(ts)), float(min(ts)), float(max(ts))
+
+    tH, tHmin, tHmax = median_time(lambda: O.omp_cheb_step(Ho, x, y), reps)
+    tV, _, _ = median_time(lambda: O.omp_mv(Vo, x, y), reps)
     S = 2048
     rng = np.random.default_rng(0)
     Ls = (rng.normal(size=(M, S)) + 1j * rng.normal(size=(M, S))) / np.sqrt(n)
     Rs = (rng.normal(size=(M, S)) + 1j * rng.normal(size=(M, S))) / np.sqrt(n)
-    t0 = time.perf_counter()
-    O.omp_bastin_ffts(Ls, Rs, nump)
-    tsite = (time.perf_counter() - t0) / S
+    tsite, _, _ = median_time(lambda: O.omp_bastin_ffts(Ls, Rs, nump), 1)
+    tsite /= S
     per_state_comp = 2 * M * tH + (M + 1) * tV + n * tsite
     step = 2 * R * per_state_comp
     return {
@@ -104,7 +143,9 @@ def cpu_baseline_bastin(host_ops, M, R, nump, phi):
         "unit": "moments/s",
         "cores": O.omp_threads(),
         "kind": "port",
-        "sample": (f"CPU restatement of Kubo_solver_FFT (OpenMP, {O.omp_threads()} threads): timed {reps} H steps "
+        "host": host_cpu(),
+        "sample": (f"CPU restatement of Kubo_solver_FFT (OpenMP, {O.omp_threads()} threads), medians of {samples} "
+                   f"samples (H step {tHmin * 1e3:.2f}-{tHmax * 1e3:.2f} ms over the samples): timed {reps} H steps "
                    f"({tH * 1e3:.2f} ms), {reps} V applies ({tV * 1e3:.2f} ms) at N={n} and the per-site Bastin "
                    f"DFT stage on {S} sites ({tsite * 1e6:.1f} us/site, nump={nump}); step extrapolated as "
                    f"2R x (2M t_H + (M+1) t_V + N t_site) = {step:.1f} s"),
@@ -139,21 +180,26 @@ def batch_instances(lq, ctx, njobs):
     return specs, mats
 
 
+_SERVE_HOST = {}
+
+
 def serve_systems(lq, ctx):
-    """name -> (H~, V_x, V_y, N, W, Ec) for the serving mix."""
+    """name -> (H~, V_x, V_y, N, W, Ec) for the serving mix, uploaded on ctx
+    (the host matrices are generated once)."""
     from linqt_amd import lattices
 
+    if not _SERVE_HOST:
+        for L in (24, 48, 96):
+            _SERVE_HOST[f"cubic{L}"] = (lattices.cubic_anderson(L, W=2.0, seed=20261207 + L), None, None, 7.01)
+        for L in (64, 256, 724):
+            _SERVE_HOST[f"kanemele{L}"] = (lattices.kane_mele(L, W=1.0, seed=20261207 + L), None, None, 4.6)
+        for L in (96, 256, 1024):
+            _SERVE_HOST[f"haldane{L}"] = (*lattices.haldane_honeycomb(L, seed=20261207 + L), 4.6)
     out = {}
-    for L in (24, 48, 96):
-        Hh = lattices.cubic_anderson(L, W=2.0, seed=20261207 + L)
-        out[f"cubic{L}"] = (lq.SparseMatrix.from_csr(ctx, Hh).Rescale(1 / 7.01, 0.0), None, None, Hh.n, 14.02, 0.0)
-    for L in (64, 256, 724):
-        Hh = lattices.kane_mele(L, W=1.0, seed=20261207 + L)
-        out[f"kanemele{L}"] = (lq.SparseMatrix.from_csr(ctx, Hh).Rescale(1 / 4.6, 0.0), None, None, Hh.n, 9.2, 0.0)
-    for L in (64, 128, 256):
-        H, Vx, Vy = lattices.haldane_honeycomb(L, seed=20261207 + L)
-        out[f"haldane{L}"] = (lq.SparseMatrix.from_csr(ctx, H).Rescale(1 / 4.6, 0.0), lq.SparseMatrix.from_csr(ctx, Vx),
-                              lq.SparseMatrix.from_csr(ctx, Vy), H.n, 9.2, 0.0)
+    for name, (H, Vx, Vy, a) in _SERVE_HOST.items():
+        out[name] = (lq.SparseMatrix.from_csr(ctx, H).Rescale(1 / a, 0.0),
+                     None if Vx is None else lq.SparseMatrix.from_csr(ctx, Vx),
+                     None if Vy is None else lq.SparseMatrix.from_csr(ctx, Vy), H.n, 2 * a, 0.0)
     return out
 
 
@@ -169,9 +215,12 @@ def cpu_baseline_batch(specs, mats):
     phi[0, 0] = 1.0
     O.omp_spectral_moments(Ho, phi, 2)
     steps = 64
-    t0 = time.perf_counter()
-    O.omp_spectral_moments(Ho, phi, steps)
-    t = time.perf_counter() - t0
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        O.omp_spectral_moments(Ho, phi, steps)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
     unit = t / (steps * (2 * Hh.nnz - Hh.n + Hh.n))
     work = sum(sp["M"] * (sp["R"] * (2 * m[1].nnz - m[1].n) + m[1].n * sp["R"]) for sp, m in zip(specs, mats))
     step = unit * work
@@ -180,8 +229,9 @@ def cpu_baseline_batch(specs, mats):
         "unit": "moments/s",
         "cores": O.omp_threads(),
         "kind": "port",
+        "host": host_cpu(),
         "sample": (f"CPU restatement of SpectralMoments (OpenMP, {O.omp_threads()} threads): 1 state x {steps} "
-                   f"moments of {specs[k]['kind']}{specs[k]['L']} (N={Hh.n}) took {t:.3f} s; extrapolated by "
+                   f"moments of {specs[k]['kind']}{specs[k]['L']} (N={Hh.n}) took {t:.3f} s (median of 5); extrapolated by "
                    f"row-step work (nnz + N per state-step) to the whole batch = {step:.1f} s"),
     }
 
@@ -192,17 +242,22 @@ def cpu_baseline_dos(host_H, M, R, phi):
     Ho = O.Op.from_csr(host_H).rescale(1 / 4.6, 0.0)
     Msample = min(M, 256)
     O.omp_spectral_moments(Ho, phi[:1], 2)
-    t0 = time.perf_counter()
-    O.omp_spectral_moments(Ho, phi[:1], Msample)
-    t = time.perf_counter() - t0
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        O.omp_spectral_moments(Ho, phi[:1], Msample)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
     per_moment_state = t / Msample
     return {
         "value": 1.0 / per_moment_state,
         "unit": "moments/s",
         "cores": O.omp_threads(),
         "kind": "port",
+        "host": host_cpu(),
         "sample": (f"CPU restatement of SpectralMoments (OpenMP, {O.omp_threads()} threads): 1 state x {Msample} "
-                   f"moments at N={host_H.n} took {t:.2f} s; rate per moment extrapolated to M={M}, R={R}"),
+                   f"moments at N={host_H.n} took {t:.3f} s (median of 5, {min(ts):.3f}-{max(ts):.3f} s); rate per "
+                   f"moment extrapolated to M={M}, R={R}"),
     }
 
 
@@ -379,6 +434,17 @@ def main():
         dt = (time.perf_counter() - t0) / reps
         out = {"calibrate": "d2d copy", "bytes_read": nbytes, "bytes_written": nbytes, "reps": reps,
                "ms": dt * 1e3, "GB/s (read+write)": 2 * nbytes / dt / 1e9}
+        ctx.free(src)
+        ctx.free(dst)
+        # the library's hand-written streaming kernels (16-byte non-temporal
+        # accesses, grid = 8 blocks per CU): 4 GiB read / copied / written
+        probe = {}
+        for kind, name in ((0, "read"), (1, "copy"), (2, "write")):
+            ms, moved = C.c_double(), C.c_double()
+            _lib.call("lq_hbm_probe", ctx.h, 4 << 30, kind, 10, C.byref(ms), C.byref(moved))
+            probe[name] = {"ms": ms.value, "GB/s": moved.value / ms.value / 1e6}
+        out["streaming_kernels"] = probe
+        out["ceiling_GB/s"] = max(v["GB/s"] for v in probe.values())
         print(json.dumps(out), flush=True)
         return
 
@@ -411,17 +477,22 @@ def main():
         from linqt_amd import serve
 
         systems = serve_systems(lq, ctx)
+        # two 2D workers, each its own context (stream) with the systems uploaded on it
+        wctx = [lq.Context(0) for _ in range(2)]
+        workers = [(c, serve_systems(lq, c)) for c in wctx]
         names_1d = [k for k, v in systems.items() if v[1] is None]
         names_2d = [k for k, v in systems.items() if v[1] is not None]
         reqs_proto = serve.request_mix(args.requests, args.rate, 20261207, names_1d, names_2d, frac_2d=0.3,
-                                       M_1d=(256, 512, 1024, 2048, 4096), M_2d=(128, 256, 512), R=(1, 2, 4, 8))
+                                       M_1d=(256, 512, 1024, 2048, 4096), M_2d=(256, 512, 1024, 2048, 4096),
+                                       R=(1, 2, 4, 8), sizes={k: v[3] for k, v in systems.items()},
+                                       max_2d_work=float(2 ** 40))
         log(f"serve: {len(reqs_proto)} requests at {args.rate} req/s over {len(systems)} systems")
         served = []
 
         def step():
             reqs = [serve.Request(r.rid, r.kind, r.system, r.M, r.R, r.arrival, r.seed) for r in reqs_proto]
-            be = serve.GpuBackend(lq, ctx, systems, chunk=32)
-            serve.Scheduler(be, hbm_budget=int(64e9)).run(reqs)
+            be = serve.GpuBackend(lq, ctx, systems, chunk=32, workers_2d=workers)
+            serve.Scheduler(be, hbm_budget=int(200e9)).run(reqs)
             be.close()
             served.append(serve.summary(reqs))
             return reqs
@@ -431,8 +502,8 @@ def main():
         M = max(r.M for r in reqs_proto)
         R = max(r.R for r in reqs_proto)
         config = {"workload": f"serve: {args.requests} Poisson requests at {args.rate:g} req/s, DOS (cubic Anderson "
-                              f"L24-96, Kane-Mele N16k-2.1M, M256-4096) + Greenwood/Bastin (Haldane N8k-131k, "
-                              f"M128-512), seed 20261207",
+                              f"L24-96, Kane-Mele N16k-2.1M, M256-4096) + Greenwood/Bastin (Haldane N18k-2.1M, "
+                              f"M256-4096 with M^2 N <= 2^40), two 2D workers beside the DOS batch, seed 20261207",
                   "requests": args.requests, "rate": args.rate, "max_N": n, "M": M, "R": R, "precision": "fp64",
                   "systems": {k: v[3] for k, v in systems.items()}}
     elif args.workload == "batch":
@@ -533,14 +604,30 @@ def main():
         "post_ms_per_step": tm["post_ms"] / tsteps,
         "instrumented_step": "one extra step after the timed ones, HIP events around every launch",
     }
-    # HBM bytes per launch of the dominant kernel, from the committed PMC summary of this workload
+    # HBM bytes per launch of the dominant kernel, from the committed PMC summary of this workload,
+    # used only when it was measured on the kernel sources of this tree
     traffic_tab = {}
+    traffic_note = None
     try:
         with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")) as f:
-            traffic_tab = json.load(f)["workloads"].get(config["workload"], {}).get("bytes_per_launch", {})
+            tab = json.load(f)
+        if tab.get("source_sha16") != kernel_source_sha16():
+            traffic_note = (f"PMC summary measured on kernel sources {tab.get('source_sha16')}, "
+                            f"this tree is {kernel_source_sha16()}: traffic omitted")
+        else:
+            traffic_tab = tab["workloads"].get(config["workload"], {}).get("bytes_per_launch", {})
     except (OSError, ValueError, KeyError):
         traffic_tab = {}
     spmm_names = sorted((k for k in traffic_tab if k.startswith("k_spmm<")), key=lambda k: -traffic_tab[k])
+    if spmm_names and tm["spmv_launches"] > 0:
+        # physical bytes of the recursion step (PMC) beside the reference-CSR algorithmic bytes
+        phys = traffic_tab[spmm_names[0]]
+        avg_ms = tm["spmv_ms"] / tm["spmv_launches"]
+        kernels["spmv"].update({"physical_kernel": spmm_names[0], "physical_bytes_per_launch": phys,
+                                "physical_GB/s": phys / (avg_ms * 1e-3) / 1e9,
+                                "frac_hbm_physical": phys / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+    kernels["spmv"]["frac_hbm_basis"] = ("frac_hbm: reference CSR bytes 20 nnz + 4(N+1) + 48 k N per launch "
+                                         "(algorithmic); frac_hbm_physical: PMC FETCH_SIZE + WRITE_SIZE bytes")
 
     def traffic_of(name):
         # the summary names template kernels with their arguments (k_x<2>)
@@ -569,7 +656,20 @@ def main():
             s_ref, s_got = np.asarray(ref[o][1]), np.asarray(result[o][1])
             accuracy[f"sigma_{comp}_rel_err"] = float(np.abs(s_got - s_ref).max() / np.abs(s_ref).max())
         log(f"accuracy vs fp64: {accuracy}")
-    if split and tm["contract_ms"] > tm["spmv_ms"]:
+    if split and PRECISIONS[args.precision] in (2, 3) and tm["contract_ms"] > tm["spmv_ms"]:
+        S = PRECISIONS[args.precision]
+        pieces = 3 if S == 2 else 6  # piece products per real product
+        launches = max(tm["contract_launches"], 1)
+        flops = 2.0 * M * M * n * R * 3 * pieces
+        avg_ms = tm["contract_ms"] / launches
+        tfs = flops / (avg_ms * 1e-3) / 1e12
+        peak = FP32_MFMA_PEAK_TFS if S == 2 else BF16_MFMA_PEAK_TFS
+        kname = f"k_contract_split<{S}>"
+        roofline = {"bound": "mfma", "kernel": kname, "achieved": tfs, "peak": peak, "unit": "TFLOP/s",
+                    "frac": tfs / peak, "traffic": traffic_of(kname),
+                    "per_launch": f"{flops:.4g} flop (2 M^2 N R x 3 real products x {pieces} piece products)",
+                    "avg_launch_ms": avg_ms}
+    elif split and tm["contract_ms"] > tm["spmv_ms"]:
         S = PRECISIONS[args.precision]
         launches = max(tm["contract_launches"], 1)
         kp = -(-n * R // 4096) * 4096
@@ -579,7 +679,7 @@ def main():
         kname = f"k_contract_i8<{S}, {2 if S <= 5 else 1}>"
         roofline = {"bound": "mfma", "kernel": kname + " (+ k_slice)", "achieved": tops,
                     "peak": INT8_MFMA_PEAK_TOPS, "unit": "TOP/s", "frac": tops / INT8_MFMA_PEAK_TOPS,
-                    "traffic": traffic_of(kname),
+                    "traffic": traffic_of(kname), "traffic_note": traffic_note,
                     "per_launch": f"{ops:.4g} int8 op (8 M^2 ceil(N R / 4096) 4096 S(S+1)/2)",
                     "avg_launch_ms": avg_ms}
     elif tm["contract_ms"] > tm["spmv_ms"]:
@@ -589,7 +689,10 @@ def main():
                else "8*M^2*N*R: four real FP64 MFMA products per complex multiply-add")
         roofline = {"bound": "mfma", "kernel": kname, "achieved": ctr_tfs, "peak": FP64_MFMA_PEAK_TFS,
                     "unit": "TFLOP/s", "frac": ctr_tfs / FP64_MFMA_PEAK_TFS, "traffic": traffic_of(kname),
+                    "traffic_note": traffic_note,
                     "per_launch": f"{per_launch:.4g} flop ({how})",
+                    "flop_executed_per_launch": per_launch,
+                    "flop_complex_equiv_per_launch": tm["contract_flops"] / max(tm["contract_launches"], 1),
                     "complex_equiv_TFLOP/s": ctr_cplx_tfs,
                     "avg_launch_ms": tm["contract_ms"] / max(tm["contract_launches"], 1)}
     else:
@@ -598,6 +701,7 @@ def main():
                     "frac": spmv_gbs / HBM_PEAK_GBS,
                     "traffic": traffic_tab[spmm_names[0]] if spmm_names else None,
                     "traffic_kernel": spmm_names[0] if spmm_names else None,
+                    "traffic_note": traffic_note,
                     "per_launch": f"{per_launch:.4g} B (reference CSR bytes: 20 nnz + 4(N+1) + 48 k N)",
                     "avg_launch_ms": tm["spmv_ms"] / max(tm["spmv_launches"], 1)}
 

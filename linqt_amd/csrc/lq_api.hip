@@ -54,6 +54,49 @@ int ctx_pinned(lq_ctx ctx, size_t bytes, void **p) {
   return LQ_OK;
 }
 
+int staged_d2h(lq_ctx ctx, const std::vector<D2H> &parts) {
+  char *stage = nullptr;
+  LQ_TRY(ctx_pinned(ctx, 2 * kStageHalf, (void **)&stage));
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  for (auto &e : ev) LQ_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  int rc = LQ_OK;
+  struct Chunk {
+    const char *src = nullptr;
+    char *dst = nullptr;
+    size_t n = 0;
+    int half = 0;
+  } pend;
+  auto drain = [&]() -> int {
+    if (!pend.n) return LQ_OK;
+    if (hipEventSynchronize(ev[pend.half]) != hipSuccess) {
+      set_error("staged copy to the host failed");
+      return LQ_ERR_HIP;
+    }
+    host_copy(pend.dst, pend.src, pend.n);
+    pend.n = 0;
+    return LQ_OK;
+  };
+  int h = 0;
+  for (const auto &p : parts) {
+    for (size_t off = 0; off < p.bytes && rc == LQ_OK; off += kStageHalf) {
+      const size_t n = std::min(kStageHalf, p.bytes - off);
+      char *buf = stage + (size_t)h * kStageHalf;
+      if (hipMemcpyAsync(buf, (const char *)p.dev + off, n, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+          hipEventRecord(ev[h], ctx->stream) != hipSuccess) {
+        set_error("staged copy to the host failed");
+        rc = LQ_ERR_HIP;
+        break;
+      }
+      rc = drain();  // the other half's chunk
+      pend = {buf, (char *)p.host + off, n, h};
+      h ^= 1;
+    }
+  }
+  if (rc == LQ_OK) rc = drain();
+  for (auto e : ev) (void)hipEventDestroy(e);
+  return rc;
+}
+
 void host_copy(void *dst, const void *src, size_t bytes) {
   const size_t chunk = (size_t)8 << 20;
   const int T = (int)std::min<size_t>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())),
@@ -737,15 +780,21 @@ size_t free_device_bytes() {
 
 // ---- captured recursion chunks ----------------------------------------------
 // A run of recursion steps captured as a HIP graph (kernel nodes only) and
-// replayed: one graph launch in place of a launch per step.  LQ_GRAPH=0
-// launches every step on its own.
-bool graphs_enabled() {
-  static const bool on = [] {
+// replayed: one graph launch in place of a launch per step.  By default only
+// the 64-step ring chunks are captured (one capture, replayed M/64 times in
+// the same solve); LQ_GRAPH=1 also captures a whole recursion loop when the
+// same solve comes again (measured at N = 65536, M = 1024, R = 8: replay
+// 7.48 ms against 7.52 ms launched, for a 2.7 ms capture), LQ_GRAPH=0 none.
+int graph_mode() {
+  static const int mode = [] {
     const char *e = getenv("LQ_GRAPH");
-    return !(e && e[0] == '0');
+    if (e && e[0] == '0') return 0;
+    if (e && e[0] == '1') return 2;
+    return 1;
   }();
-  return on;
+  return mode;
 }
+bool graphs_enabled() { return graph_mode() > 0; }
 
 // Captures the launches `body` makes on the context's stream into a graph,
 // or returns the one cached under the same key (operator and its version,
@@ -1915,18 +1964,20 @@ int lq_spectral_moments(lq_csr H, lq_csr OP, const double *phi, int R, int M, in
       LQ_TRY(launch_copy(ctx->stream, A, phi_d, vec));
     }
     if (algo == LQ_ALGO_DIRECT) {
-      // Moment m's block partials go to slot m % kPartRing of a ring; each
-      // full ring (and the last, partial one) is reduced into sums0 before
-      // the ring is reused, so partials take kPartRing slots whatever M is.
-      // Every slot is reduced by itself in the same order as before.
-      LQ_TRY(ctx_buf(ctx, "part0", (size_t)kPartRing * grid * kp * sizeof(double2), (void **)&part0));
-      LQ_TRY(ctx_buf(ctx, "sums0", (size_t)M * kp * sizeof(double2), (void **)&sums0));
+      // Moment m's block partials go to slot m % RS of the partial buffer:
+      // RS = M when all moments' partials fit in kPartBytes (one reduction at
+      // the end), else a ring of kPartRing slots, reduced into sums0 each time
+      // it fills.  Every slot is reduced by itself in the same order either way.
       const size_t slot = (size_t)grid * kp;
-      auto flush = [&](int m_end) -> int {  // moments [m_end - filled, m_end) are in the ring
-        const int m0 = ((m_end - 1) / kPartRing) * kPartRing;
+      const bool whole = (size_t)M * slot * sizeof(double2) <= kPartBytes;
+      const int RS = whole ? M : kPartRing;
+      LQ_TRY(ctx_buf(ctx, "part0", (size_t)RS * slot * sizeof(double2), (void **)&part0));
+      LQ_TRY(ctx_buf(ctx, "sums0", (size_t)M * kp * sizeof(double2), (void **)&sums0));
+      auto flush = [&](int m_end) -> int {  // moments [m_end - filled, m_end) are in the buffer
+        const int m0 = ((m_end - 1) / RS) * RS;
         return launch_reduce_partials(ctx->stream, part0, m_end - m0, grid, kp, sums0 + (size_t)m0 * kp);
       };
-      auto ring = [&](int m) { return part0 + (size_t)(m % kPartRing) * slot; };
+      auto ring = [&](int m) { return part0 + (size_t)(m % RS) * slot; };
       LQ_TRY(launch_dot(ctx->stream, n, gg, phi_d, A, ring(0), nullptr));
       SpmvLoopGuard guard{ctx};
       LQ_TRY(spmv_loop_begin(ctx));
@@ -1934,38 +1985,67 @@ int lq_spectral_moments(lq_csr H, lq_csr OP, const double *phi, int R, int M, in
         VecSet s{A, nullptr, B, gg};
         LQ_TRY(spmm(ctx, H, one, zero, &s, 1, kDotPhi, phi_d, ring(1)));
       }
-      // One full ring of steps (moments 64c .. 64c + 63) always starts from
-      // the same A/B assignment (an even number of swaps before it) and
-      // writes ring slots 0..63: with graphs on it is captured once and
-      // replayed, the ring reduction launched between replays.
-      hipGraphExec_t chunk_graph = nullptr;
-      const int graph_from = graphs_enabled() ? kPartRing : M;  // first moment produced by a replay
-      for (int m = 1; m + 1 < M; ++m) {
-        if ((m + 1) % kPartRing == 0) LQ_TRY(flush(m + 1));
-        if (m + 1 >= graph_from && (m + 1) % kPartRing == 0 && m + kPartRing < M) {
-          if (!chunk_graph) {
-            GraphKey key{H->uid, H->version, A, B, phi_d, part0, n, gg, kPartRing};
-            LQ_TRY(recursion_chunk_graph(ctx, key, [&]() -> int {
-              double2 *a = A, *b = B;
-              for (int q = 0; q < kPartRing; ++q) {
-                VecSet s{b, a, a, gg};
-                LQ_TRY(spmm(ctx, H, two, mone, &s, 1, kDotPhi, phi_d, part0 + (size_t)q * slot));
-                std::swap(a, b);
-              }
-              return LQ_OK;
-            }, &chunk_graph));
-          }
-          LQ_HIP_CHECK(hipGraphLaunch(chunk_graph, ctx->stream));
-          ctx->loop_launches += kPartRing;
-          ctx->loop_bytes += (double)kPartRing * spmv_bytes(H, gg, true);
-          m += kPartRing - 1;  // an even count of steps: A and B end where they started
-          continue;
+      // Captured recursion (graphs on): with all partials in one buffer the
+      // whole loop m = 1 .. M-2 is one graph, captured when the same solve
+      // (operator, buffers, sizes) comes again and replayed from then on;
+      // with the ring, one full ring of steps (moments 64c .. 64c + 63)
+      // always starts from the same A/B assignment (an even number of swaps
+      // before it) and writes slots 0..63, so it is captured once and
+      // replayed with the ring reduction between replays.
+      const int nsteps = M - 2;
+      if (whole && graph_mode() == 2 && nsteps >= 2) {
+        const GraphKey key{H->uid, H->version, A, B, phi_d, part0, n, gg, nsteps};
+        hipGraphExec_t g = nullptr;
+        if (ctx->graph_seen == key || (ctx->graph.exec && ctx->graph.key == key)) {
+          LQ_TRY(recursion_chunk_graph(ctx, key, [&]() -> int {
+            double2 *a = A, *b = B;
+            for (int m = 1; m + 1 < M; ++m) {
+              VecSet s{b, a, a, gg};
+              LQ_TRY(spmm(ctx, H, two, mone, &s, 1, kDotPhi, phi_d, ring(m + 1)));
+              std::swap(a, b);
+            }
+            return LQ_OK;
+          }, &g));
+          LQ_HIP_CHECK(hipGraphLaunch(g, ctx->stream));
+          ctx->loop_launches += nsteps;
+          ctx->loop_bytes += (double)nsteps * spmv_bytes(H, gg, true);
+          if (nsteps % 2) std::swap(A, B);
+        } else {
+          ctx->graph_seen = key;  // a repeat of this solve is captured
         }
-        // A <- 2 H~ B - A, then swap: after the step B holds T_{m+1}
-        VecSet s{B, A, A, gg};
-        LQ_TRY(spmm(ctx, H, two, mone, &s, 1, kDotPhi, phi_d, ring(m + 1)));
-        std::swap(A, B);
+        if (g) goto recursion_done;
       }
+      {
+        hipGraphExec_t chunk_graph = nullptr;
+        const int graph_from = graphs_enabled() && !whole ? kPartRing : M;  // first moment a replay produces
+        for (int m = 1; m + 1 < M; ++m) {
+          if ((m + 1) % RS == 0) LQ_TRY(flush(m + 1));
+          if (m + 1 >= graph_from && (m + 1) % kPartRing == 0 && m + kPartRing < M) {
+            if (!chunk_graph) {
+              GraphKey key{H->uid, H->version, A, B, phi_d, part0, n, gg, -kPartRing};  // < 0: a ring chunk
+              LQ_TRY(recursion_chunk_graph(ctx, key, [&]() -> int {
+                double2 *a = A, *b = B;
+                for (int q = 0; q < kPartRing; ++q) {
+                  VecSet s{b, a, a, gg};
+                  LQ_TRY(spmm(ctx, H, two, mone, &s, 1, kDotPhi, phi_d, part0 + (size_t)q * slot));
+                  std::swap(a, b);
+                }
+                return LQ_OK;
+              }, &chunk_graph));
+            }
+            LQ_HIP_CHECK(hipGraphLaunch(chunk_graph, ctx->stream));
+            ctx->loop_launches += kPartRing;
+            ctx->loop_bytes += (double)kPartRing * spmv_bytes(H, gg, true);
+            m += kPartRing - 1;  // an even count of steps: A and B end where they started
+            continue;
+          }
+          // A <- 2 H~ B - A, then swap: after the step B holds T_{m+1}
+          VecSet s{B, A, A, gg};
+          LQ_TRY(spmm(ctx, H, two, mone, &s, 1, kDotPhi, phi_d, ring(m + 1)));
+          std::swap(A, B);
+        }
+      }
+    recursion_done:
       LQ_TRY(spmv_loop_end(ctx));
       hipEvent_t ev;
       LQ_TRY(time_begin(ctx, kCatPost, &ev));

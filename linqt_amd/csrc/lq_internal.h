@@ -54,6 +54,9 @@ constexpr int kSlice = 64;
 // Slots of the ring that holds per-block dot partials of the 1D recursion
 // (one slot per moment or step); a full ring is reduced before it is reused.
 constexpr int kPartRing = 64;
+// Device bytes up to which the 1D solver keeps every moment's partials (one
+// reduction at the end instead of a ring).
+constexpr size_t kPartBytes = (size_t)256 << 20;
 
 // Kernel-time categories recorded with HIP events.
 enum TimeCat { kCatSpmv = 0, kCatContract = 1, kCatPost = 2, kNumCats = 3 };
@@ -286,6 +289,7 @@ struct lq_ctx_s {
     hipGraphExec_t exec = nullptr;
     GraphKey key;
   } graph;
+  GraphKey graph_seen;  // the last whole-loop solve run without a graph
 };
 
 // Host-side record of the completed operator (kept for rescale and for
@@ -343,6 +347,16 @@ int ctx_buf(lq_ctx ctx, const char *name, size_t bytes, void **p);
 int ctx_pinned(lq_ctx ctx, size_t bytes, void **p);
 // dst[0, bytes) = src[0, bytes) on up to 16 host threads.
 void host_copy(void *dst, const void *src, size_t bytes);
+// Device -> pageable host copies through the context's pinned staging buffer,
+// 2 x kStageHalf bytes whatever the sizes, chunk copies overlapping host
+// threads (the stream must be idle or ordered before).
+constexpr size_t kStageHalf = (size_t)32 << 20;
+struct D2H {
+  const void *dev;
+  void *host;
+  size_t bytes;
+};
+int staged_d2h(lq_ctx ctx, const std::vector<D2H> &parts);
 int time_begin(lq_ctx ctx, int cat, hipEvent_t *ev);
 int time_end(lq_ctx ctx, int cat, hipEvent_t ev, double amount, int launches = 1);
 // The same on another stream of the context (stream2).

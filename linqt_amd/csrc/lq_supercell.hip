@@ -362,28 +362,19 @@ extern "C" int lq_csr_from_hoppings(lq_ctx ctx, int nwann, int nhop, const int *
     A->col.resize((size_t)total);
     A->val.resize((size_t)total);
     A->kind.resize((size_t)total);
-    // through the context's pinned staging buffer: one fast copy from the
-    // device, then host threads into the operator's arrays
+    // through the context's pinned staging buffer (two halves of kStageHalf
+    // bytes whatever the operator's size): the copy of one chunk from the
+    // device overlaps host threads moving the previous chunk into the
+    // operator's arrays
     const size_t bc = sizeof(int) * (size_t)total, bv = sizeof(double2) * (size_t)total, bk = (size_t)total;
-    char *stage = nullptr;
-    if (total > 0 && (rc = ctx_pinned(ctx, bc + bv + bk, (void **)&stage)) != LQ_OK) break;
-    if (total > 0 &&
-        (hipMemcpyAsync(stage, d_col, bc, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-         hipMemcpyAsync(stage + bc, d_val, bv, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-         hipMemcpyAsync(stage + bc + bv, d_kind, bk, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)) {
-      set_error("lq_csr_from_hoppings: copying the operator failed");
-      rc = LQ_ERR_HIP;
-      break;
-    }
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
       set_error("lq_csr_from_hoppings: generator kernels failed");
       rc = LQ_ERR_HIP;
       break;
     }
     if (total > 0) {
-      host_copy(A->col.data(), stage, bc);
-      host_copy(A->val.data(), stage + bc, bv);
-      host_copy(A->kind.data(), stage + bc + bv, bk);
+      rc = staged_d2h(ctx, {{d_col, A->col.data(), bc}, {d_val, A->val.data(), bv}, {d_kind, A->kind.data(), bk}});
+      if (rc != LQ_OK) break;
     }
     dsrc.rowptr = d_rowptr;
     dsrc.col = d_col;

@@ -97,6 +97,12 @@ int lq_abi_version(void);
 int lq_device_count(int *count);
 int lq_ctx_create(int device, lq_ctx *out);
 int lq_ctx_destroy(lq_ctx ctx);
+/* No counterpart in the reference (its buffers live as long as its objects).
+ * Waits for the context's work and frees the device scratch buffers, pinned
+ * staging and captured graphs it keeps between calls (they grow to the
+ * largest solve run on it); operators and the context stay valid.  A server
+ * calls it between requests so a context holds no memory while idle. */
+int lq_ctx_release(lq_ctx ctx);
 int lq_ctx_synchronize(lq_ctx ctx);
 int lq_ctx_set_timing(lq_ctx ctx, int enabled);
 int lq_ctx_reset_timing(lq_ctx ctx);

@@ -83,6 +83,10 @@ class Context:
     def __exit__(self, *exc):
         self.close()
 
+    def release(self):
+        """Frees the device scratch buffers the context keeps between calls."""
+        _lib.call("lq_ctx_release", self.h)
+
     def synchronize(self):
         _lib.call("lq_ctx_synchronize", self.h)
 

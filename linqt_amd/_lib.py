@@ -72,6 +72,7 @@ _SIGS = {
     "lq_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "lq_ctx_destroy": (C.c_int, [_vp]),
     "lq_ctx_synchronize": (C.c_int, [_vp]),
+    "lq_ctx_release": (C.c_int, [_vp]),
     "lq_ctx_set_timing": (C.c_int, [_vp, C.c_int]),
     "lq_ctx_reset_timing": (C.c_int, [_vp]),
     "lq_ctx_get_timing": (C.c_int, [_vp, C.POINTER(lq_timing)]),

@@ -1464,6 +1464,24 @@ int lq_ctx_destroy(lq_ctx ctx) try {
   return LQ_OK;
 } LQ_ABI_CATCH
 
+int lq_ctx_release(lq_ctx ctx) try {
+  if (!ctx) return LQ_ERR_ARG;
+  DeviceGuard g(ctx->device);
+  LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  if (ctx->stream2) LQ_HIP_CHECK(hipStreamSynchronize(ctx->stream2));
+  LQ_TRY(collect_timing(ctx));
+  for (auto &kv : ctx->bufs)
+    if (kv.second.p) LQ_HIP_CHECK(hipFree(kv.second.p));
+  ctx->bufs.clear();
+  if (ctx->pinned.p) LQ_HIP_CHECK(hipHostFree(ctx->pinned.p));
+  ctx->pinned = {};
+  if (ctx->graph.exec) LQ_HIP_CHECK(hipGraphExecDestroy(ctx->graph.exec));
+  ctx->graph.exec = nullptr;
+  ctx->graph.key = GraphKey{};
+  ctx->graph_seen = GraphKey{};
+  return LQ_OK;
+} LQ_ABI_CATCH
+
 int lq_ctx_synchronize(lq_ctx ctx) try {
   if (!ctx) return LQ_ERR_ARG;
   DeviceGuard g(ctx->device);

@@ -215,11 +215,12 @@ class GpuBackend:
         n = self.systems[r.system][3]
         if r.kind == "dos":
             return self.lq.Batch.job_bytes(n, r.R, r.M, self.chunk)
-        # the 2D pipeline's buffers for one group of R states: the left vectors,
-        # two blocks of right products, work vectors (the streamed pipeline
-        # bounds this when M N is large; the resident one stores M right
-        # vectors per operator)
-        return min((3 * r.M + 4), (r.M + 4 * 128 + 5)) * n * r.R * 16 if r.M > 512 else (2 * r.M + 4) * n * r.R * 16
+        # the resident 2D pipeline of one operator (sigma_xx) for all R states in
+        # one group: M left and M right vectors, phi and two work vectors per
+        # state, plus split-K partial tables; the worker releases it afterwards,
+        # so with the budget below the device's memory every admitted request
+        # runs the same grouping, and gives the same result, as when alone
+        return (2 * r.M + 3) * n * r.R * 16 + (256 << 20)
 
     def start_1d(self, r: Request):
         self.jobs[self.batch.submit(self.systems[r.system][0], self.states(r), r.M)] = r.rid
@@ -252,6 +253,7 @@ class GpuBackend:
                 return
             try:
                 res = self._solve_2d(ctx, systems, r)
+                ctx.release()  # the context holds no buffers between requests
             except Exception as e:  # reported by poll_2d's caller through the result
                 res = e
             with self._cv:

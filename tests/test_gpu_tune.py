@@ -48,3 +48,21 @@ def test_tuned_schedules_keep_results_and_persist(ctx, tmp_path, monkeypatch):
     assert len(lines) == 3
     with pytest.raises(lq.LinqtError):
         H.tune(0)
+
+
+def test_ctx_release_then_solve_again(ctx):
+    """lq_ctx_release frees the context's buffers and cached graphs; the next
+    solves allocate again and give the same moments and tables."""
+    S = lattices.square_anderson(32, W=1.0, seed=2)
+    H = lq.SparseMatrix.from_csr(ctx, S).Rescale(1 / 4.6, 0.0)
+    phi = lattices.random_phase_states(S.n, 4, seed=3)
+    a = lq.spectral_moments(H, None, phi, 200)
+    Hh, Vx, Vy = lattices.haldane_honeycomb(12, seed=4)
+    G = lq.SparseMatrix.from_csr(ctx, Hh).Rescale(1 / 4.6, 0.0)
+    V = lq.SparseMatrix.from_csr(ctx, Vx)
+    ph = lattices.random_phase_states(Hh.n, 2, seed=5)
+    t = lq.kubo_fft(G, V, V, ph, 48, 128, lq.LQ_KUBO_BASTIN)
+    ctx.release()
+    ctx.release()  # twice is harmless
+    assert np.array_equal(lq.spectral_moments(H, None, phi, 200), a)
+    assert np.array_equal(lq.kubo_fft(G, V, V, ph, 48, 128, lq.LQ_KUBO_BASTIN), t)

@@ -84,13 +84,46 @@ struct lq_batch_s {
   std::map<int, Tune> tune;
   int fixed_unroll = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // Device blocks of finished jobs kept for reuse (best fit), so admitting and
+  // harvesting jobs calls hipMalloc only when no kept block fits and hipFree
+  // (which waits for the whole device, other streams included) only when more
+  // than kKeepBytes would be kept.
+  std::multimap<size_t, void *> spare;
+  size_t spare_bytes = 0;
 };
 
 namespace {
 
+constexpr size_t kKeepBytes = (size_t)4 << 30;
+
+int block_get(lq_batch b, size_t bytes, void **p) {
+  auto it = b->spare.lower_bound(bytes);
+  if (it != b->spare.end() && it->first <= 2 * bytes + ((size_t)64 << 20)) {  // best fit, not wasteful
+    *p = it->second;
+    b->spare_bytes -= it->first;
+    b->spare.erase(it);
+    return LQ_OK;
+  }
+  LQ_HIP_CHECK(hipMalloc(p, bytes));
+  return LQ_OK;
+}
+
+int block_put(lq_batch b, void *p, size_t bytes) {
+  b->spare.emplace(bytes, p);
+  b->spare_bytes += bytes;
+  while (b->spare_bytes > kKeepBytes && !b->spare.empty()) {  // drop the largest kept blocks
+    auto it = std::prev(b->spare.end());
+    b->spare_bytes -= it->first;
+    void *q = it->second;
+    b->spare.erase(it);
+    LQ_HIP_CHECK(hipFree(q));
+  }
+  return LQ_OK;
+}
+
 int admit(lq_batch b, Job &j) {
   lq_ctx ctx = b->ctx;
-  LQ_HIP_CHECK(hipMalloc(&j.mem, j.bytes));
+  LQ_TRY(block_get(b, j.bytes, &j.mem));
   char *p = (char *)j.mem;
   const size_t vec = (size_t)j.n * j.R * sizeof(double2);
   BatchJobDev &d = j.d;
@@ -134,10 +167,11 @@ int harvest(lq_batch b, Job &j) {
     if (m == 0) scal *= 0.5;
     for (int v = 0; v < j.R; ++v) j.result[m] += scal * std::complex<double>(h[(size_t)m * j.kp + v].x, h[(size_t)m * j.kp + v].y);
   }
-  LQ_HIP_CHECK(hipFree(j.mem));
+  void *m = j.mem;
   j.mem = nullptr;
   b->used -= j.bytes;
   j.state = 2;
+  LQ_TRY(block_put(b, m, j.bytes));
   return LQ_OK;
 }
 
@@ -285,6 +319,7 @@ int lq_batch_destroy(lq_batch b) try {
   (void)hipStreamSynchronize(b->ctx->stream);
   for (auto &kv : b->jobs)
     if (kv.second.mem) (void)hipFree(kv.second.mem);
+  for (auto &kv : b->spare) (void)hipFree(kv.second);
   for (auto &kv : b->cls)
     if (kv.second.dev) (void)hipFree(kv.second.dev);
   if (b->tc) (void)hipFree(b->tc);

@@ -92,7 +92,9 @@ class Scheduler:
     One loop: admit arrived requests (DOS into the batch, 2D to a free
     worker), advance the batch by one chunk, collect finished 2D solves.  The
     loop never waits on a 2D solve; with nothing to step it sleeps until the
-    next arrival or until a 2D solve finishes."""
+    next arrival or until a 2D solve finishes.  Memory: each 2D worker holds
+    the largest 2D footprint of the stream (reserved up front), DOS requests
+    share the rest of the budget."""
 
     def __init__(self, backend, hbm_budget: int, clock: Callable[[], float] | None = None,
                  sleep: Callable[[float], None] | None = None):
@@ -105,9 +107,16 @@ class Scheduler:
 
     def run(self, requests: list[Request]) -> list[Request]:
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        # every 2D worker keeps the buffers of the largest 2D request it may run
+        # (no allocation churn between requests), so that much is reserved per
+        # worker up front and DOS requests are admitted into the rest
+        fp2d = [self.b.footprint(r) for r in pending if r.kind != "dos"]
+        self.reserved = max(fp2d, default=0) * (self.b.slots_2d if fp2d else 0)
         for r in pending:
-            if self.b.footprint(r) > self.budget:
-                raise ValueError(f"request {r.rid} needs {self.b.footprint(r)} bytes, above the budget")
+            fp = self.b.footprint(r)
+            if (r.kind == "dos" and fp > self.budget - self.reserved) or fp > self.budget:
+                raise ValueError(f"request {r.rid} needs {fp} bytes, above the budget "
+                                 f"({self.reserved} reserved for 2D workers)")
         waiting: list[Request] = []
         running_1d: dict[int, Request] = {}
         running_2d: dict[int, Request] = {}
@@ -116,7 +125,8 @@ class Scheduler:
 
         def finish(r):
             r.finish = now()
-            self.used -= self.b.footprint(r)
+            if r.kind == "dos":
+                self.used -= self.b.footprint(r)
             self.log.append((r.finish, "finish", r.rid))
 
         while pending or waiting or running_1d or running_2d:
@@ -124,13 +134,14 @@ class Scheduler:
                 waiting.append(pending.pop(0))
             # first-fit admission in arrival order: DOS into the batch, 2D to a free worker
             for r in list(waiting):
-                fp = self.b.footprint(r)
-                if self.used + fp > self.budget:
-                    continue
-                if r.kind != "dos" and len(running_2d) >= self.b.slots_2d:
+                if r.kind == "dos":
+                    fp = self.b.footprint(r)
+                    if self.used + fp > self.budget - self.reserved:
+                        continue
+                    self.used += fp
+                elif len(running_2d) >= self.b.slots_2d:
                     continue
                 waiting.remove(r)
-                self.used += fp
                 r.start = now()
                 self.log.append((r.start, "start", r.rid))
                 if r.kind == "dos":
@@ -217,9 +228,10 @@ class GpuBackend:
             return self.lq.Batch.job_bytes(n, r.R, r.M, self.chunk)
         # the resident 2D pipeline of one operator (sigma_xx) for all R states in
         # one group: M left and M right vectors, phi and two work vectors per
-        # state, plus split-K partial tables; the worker releases it afterwards,
-        # so with the budget below the device's memory every admitted request
-        # runs the same grouping, and gives the same result, as when alone
+        # state, plus split-K partial tables; a worker's buffers stay at its
+        # largest request, which the scheduler reserves, so with the budget
+        # below the device's memory every request runs the same grouping, and
+        # gives the same result, as when alone
         return (2 * r.M + 3) * n * r.R * 16 + (256 << 20)
 
     def start_1d(self, r: Request):
@@ -253,7 +265,6 @@ class GpuBackend:
                 return
             try:
                 res = self._solve_2d(ctx, systems, r)
-                ctx.release()  # the context holds no buffers between requests
             except Exception as e:  # reported by poll_2d's caller through the result
                 res = e
             with self._cv:

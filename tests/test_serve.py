@@ -100,6 +100,7 @@ def test_scheduler_completes_all_within_budget():
     assert all(r.finish is not None and r.result is not None for r in out)
     assert all(r.start >= r.arrival - 1e-12 and r.finish >= r.start for r in out)
     assert be.peak <= budget
+    assert sched.reserved == 2 * max(be.footprint(r) for r in reqs if r.kind != "dos")
     # 2D requests start in arrival order (each fits the budget next to the batch here)
     twod = [rid for rid in be.starts if out[rid].kind != "dos"]
     assert twod == sorted(twod, key=lambda i: (out[i].arrival, i))

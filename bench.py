@@ -63,7 +63,8 @@ def kernel_source_sha16():
 
     h = hashlib.sha256()
     root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "linqt_amd", "csrc")
-    for path in sorted(glob.glob(os.path.join(root, "*.hip")) + glob.glob(os.path.join(root, "*.h"))):
+    for path in sorted(glob.glob(os.path.join(root, "*.hip")) + glob.glob(os.path.join(root, "*.h"))
+                       + glob.glob(os.path.join(root, "modules", "*.hip"))):
         h.update(os.path.basename(path).encode())
         with open(path, "rb") as f:
             h.update(f.read())
@@ -585,14 +586,19 @@ def main():
     # kernel (tables of at least 128 x 128) does 6 per complex multiply-add
     ctr_cplx_tfs = tm["contract_flops"] / (tm["contract_ms"] * 1e-3) / 1e12 if tm["contract_ms"] > 0 else 0.0
     ctr_mode = os.environ.get("LQ_CONTRACT", "")
+    ctr_128x64 = ctr_mode in ("dl", "w1") or "LQ_CONTRACT_DL" in os.environ or "LQ_CONTRACT_DLBUF" in os.environ
     if ctr_mode == "4m":
         ctr_kernel, ctr_fpc = "k_contract_f64_lds", 8
     elif ctr_mode == "wave" or config.get("M", 0) < 128:
         ctr_kernel, ctr_fpc = "k_contract_f64", 8
-    elif os.environ.get("LQ_CONTRACT_DL", "") == "0":
+    elif ctr_mode == "w1":
+        ctr_kernel, ctr_fpc = "k_contract_f64_3m_w1", 6
+    elif ctr_128x64 and os.environ.get("LQ_CONTRACT_DL", "") == "0":
         ctr_kernel, ctr_fpc = "k_contract_f64_3m", 6
-    else:
+    elif ctr_128x64:
         ctr_kernel, ctr_fpc = "k_contract_f64_3m_dl", 6
+    else:
+        ctr_kernel, ctr_fpc = "lq_contract_f64_3m_t128", 6
     ctr_tfs = ctr_cplx_tfs * ctr_fpc / 8
     kernels = {
         "spmv": {"ms_per_step": tm["spmv_ms"] / tsteps, "launches_per_step": tm["spmv_launches"] / tsteps,

@@ -6,31 +6,32 @@
 // table is one complex GEMM  C = A^H B  on FP64 MFMA (v_mfma_f64_16x16x4_f64):
 //   Re C = Ar Br + Ai Bi,   Im C = Ar Bi - Ai Br.
 //
-// Kernels: k_contract_f64_3m_dl (the default for tables of at least
-// 128 x 128: three real products per complex product, 128 x 64 block tile,
-// operands copied from global memory straight into LDS buffers),
-// k_contract_f64_3m (the same products staged through registers into two LDS
-// buffers, LQ_CONTRACT_DL=0), k_contract_f64_lds (the four-product form with
-// a 128 x 128 tile, LQ_CONTRACT=4m) and k_contract_f64 (64 x 64 tile,
-// operands straight from global memory, for small tables).  The table is a
+// Kernels: lq_contract_f64_3m_t128 (the default for tables of at least
+// 128 x 128: three real products per complex product, 128 x 128 block tile,
+// operands copied from global memory straight into LDS buffers; built as a
+// code object of its own from modules/lq_contract_t128.hip and loaded here),
+// k_contract_f64_3m_dl (the same with a 128 x 64 tile, LQ_CONTRACT=dl),
+// k_contract_f64_3m_w1 (its one-wave-per-SIMD form, LQ_CONTRACT=w1),
+// k_contract_f64_3m (the 128 x 64 products staged through registers,
+// LQ_CONTRACT_DL=0), k_contract_f64_lds (the four-product form with a
+// 128 x 128 tile, LQ_CONTRACT=4m) and k_contract_f64 (64 x 64 tile, operands
+// straight from global memory, for small tables).  The table is a
 // GEMM with a tiny output and a huge K: the LDS kernels read each operand tile
 // once per block, K is split into nsplit chunks, and the blocks of one chunk
 // run together on one XCD (block_chunk_tile).  A deterministic second pass
 // sums the chunks.
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 
+#include "lq_contract_dev.h"
 #include "lq_internal.h"
 
 namespace lq {
 
 namespace {
 
-typedef double v4d __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
 
 // One K step of 16 complex elements: lane (r16, ks) feeds the k-slot ks of
 // four consecutive MFMAs with elements kk + s, s = 0..3.  A and B use the same
@@ -50,24 +51,6 @@ __device__ __forceinline__ void mma_step(const double2 (&a)[2][4], const double2
       }
 }
 
-// Block -> (K chunk, output tile).  Blocks are dispatched round-robin over
-// the 8 XCDs (block b on XCD b % 8), each with its own L2.  xmap (nsplit a
-// multiple of 8): XCD x takes chunks x, x + 8, ... one after another, all
-// tiles of a chunk together, so every row of A and B in the chunk is read
-// from HBM once and then shared through that XCD's L2.  Otherwise
-// chunk = b % nsplit.  Either way each (chunk, tile) is one block and writes
-// the same partial table, so the result does not depend on the mapping.
-__device__ __forceinline__ void block_chunk_tile(int bid, int nsplit, int xmap, int &chunk, int &tile) {
-  if (xmap) {
-    const int tiles = (int)gridDim.x / nsplit;
-    const int j = bid >> 3;
-    chunk = (bid & 7) + 8 * (j / tiles);
-    tile = j % tiles;
-  } else {
-    chunk = bid % nsplit;
-    tile = bid / nsplit;
-  }
-}
 
 __global__ __launch_bounds__(256) void k_contract_f64(const double2 *__restrict__ A, long long lda, int M0,
                                                       const double2 *__restrict__ B, long long ldb, int M1,
@@ -166,7 +149,6 @@ __global__ __launch_bounds__(256) void k_contract_f64(const double2 *__restrict_
 // buffer feeds the MFMAs.  Rows are padded to 17 elements so the 16 rows read
 // by one MFMA fragment land in distinct LDS banks.
 constexpr int kBT = 128;       // block tile (rows of A and of B)
-constexpr int kKC = 16;        // complex elements per stage
 constexpr int kLdsRow = kKC + 1;
 
 __global__ __launch_bounds__(512) void k_contract_f64_lds(const double2 *__restrict__ A, long long lda, int M0,
@@ -424,7 +406,6 @@ __global__ __launch_bounds__(512) void k_contract_f64_3m(const double2 *__restri
 // zeros past K into the same slots.
 constexpr int kDlA = kBM3 * kKC, kDlB = kBN3 * kKC;  // slots per buffer
 
-__device__ __forceinline__ int dl_slot(int row, int k) { return row * kKC + (k ^ (row & 15)); }
 
 __device__ __forceinline__ void mma3_stage_dl(const double2 *sA, const double2 *sB, int arow0, int brow0, int ks,
                                               v4d (&p1)[2][2], v4d (&p2)[2][2], v4d (&p3)[2][2]) {
@@ -456,7 +437,6 @@ __device__ __forceinline__ void mma3_stage_dl(const double2 *sA, const double2 *
   }
 }
 
-typedef __attribute__((address_space(3))) void lds_void_t;
 
 // One global_load_lds_dwordx4: lane l copies 16 bytes from g (its own
 // address) to LDS byte address lds + 16 l (lds uniform over the wave).
@@ -593,6 +573,142 @@ __global__ __launch_bounds__(512) void k_contract_f64_3m_dl(const double2 *__res
       }
 }
 
+// One wave per SIMD: the same block tile (128 x 64), LDS layout, direct loads
+// and MFMA sequence per output element as k_contract_f64_3m_dl, with four
+// waves of 64 x 32 (ti = 0..3, tj = 0..1) instead of eight of 32 x 32.  Each
+// wave does twice the MFMA work between two barriers, and its 96 accumulator
+// registers x 2 live in AGPRs, so a block holds about 3/5 of a SIMD lane's
+// registers and leaves room for a wave of another kernel (the recursion
+// step) on every SIMD.  Each output element sees the same products in the
+// same K order as in k_contract_f64_3m_dl: the tables are bitwise equal.
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_num_vgpr(192))) void k_contract_f64_3m_w1(const double2 *__restrict__ A, long long lda, int M0,
+                                                            const double2 *__restrict__ B, long long ldb, int M1,
+                                                            long long K, long long kchunk, int nsplit, int tiles_n,
+                                                            int xmap, double2 *__restrict__ work) {
+  // buffer b: A slots at sbuf + b (kDlA + kDlB), then the B slots
+  __shared__ double2 sbuf[2 * (kDlA + kDlB)];
+  int chunk, tile;
+  block_chunk_tile(blockIdx.x, nsplit, xmap, chunk, tile);
+  const int tm = tile / tiles_n, tn = tile % tiles_n;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int r16 = lane & 15, ks = lane >> 4;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 32;
+
+  // Load q of this wave covers slots 64 c .. 64 c + 63 of a buffer (c = 8 wave + q
+  // for A, 4 wave + q for B): lane l fills slot 64 c + l = row R = 4 c + l / 16,
+  // element (l % 16) ^ (R & 15).
+  const int lrow = lane >> 4, lk = lane & 15;
+  const double2 *pa[8];
+  const double2 *pb[4];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int R = 4 * (8 * wave + q) + lrow;
+    int g = tm * kBM3 + R;
+    if (g >= M0) g = M0 - 1;  // clamped rows feed outputs that are never stored
+    pa[q] = A + (size_t)g * lda + (lk ^ (R & 15));
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int R = 4 * (4 * wave + q) + lrow;
+    int g = tn * kBN3 + R;
+    if (g >= M1) g = M1 - 1;
+    pb[q] = B + (size_t)g * ldb + (lk ^ (R & 15));
+  }
+
+  const long long k0 = (long long)chunk * kchunk;
+  const long long len = (k0 + kchunk < K ? k0 + kchunk : K) - k0;
+  const int nfull = len > 0 ? (int)(len / kKC) : 0;              // whole stages
+  const int nstage = len > 0 ? (int)((len + kKC - 1) / kKC) : 0;  // nfull or nfull + 1
+  auto abuf = [&](int j) { return sbuf + (j & 1) * (kDlA + kDlB); };
+
+  // A whole stage: twelve direct loads per lane.
+  auto issue_full = [&](int j) {
+    const long long kb = k0 + (long long)j * kKC;
+    double2 *sA = abuf(j), *sB = sA + kDlA;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) lds_dma16(pa[q] + kb, sA + 64 * (8 * wave + q));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) lds_dma16(pb[q] + kb, sB + 64 * (4 * wave + q));
+  };
+  // The partial last stage: through registers, zeros past the chunk end (the
+  // element of a slot is its pointer's offset from the row start, lk ^ (R & 15)).
+  auto issue_part = [&](int j) {
+    const long long kb = (long long)j * kKC;
+    double2 *sA = abuf(j), *sB = sA + kDlA;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int R = 4 * (8 * wave + q) + lrow;
+      sA[64 * (8 * wave + q) + lane] = kb + (lk ^ (R & 15)) < len ? pa[q][k0 + kb] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int R = 4 * (4 * wave + q) + lrow;
+      sB[64 * (4 * wave + q) + lane] = kb + (lk ^ (R & 15)) < len ? pb[q][k0 + kb] : make_double2(0.0, 0.0);
+    }
+  };
+  v4d p1[4][2], p2[4][2], p3[4][2];
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) {
+      p1[ti][tj] = p2[ti][tj] = p3[ti][tj] = v4d{0.0, 0.0, 0.0, 0.0};
+      asm("" : "+a"(p1[ti][tj]), "+a"(p2[ti][tj]), "+a"(p3[ti][tj]));
+    }
+
+  // This wave's loads of the next stage complete and every LDS read of the
+  // current one has returned, then a barrier without a memory fence.
+  auto settle = [&]() { asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory"); };
+  // The empty asm statements keep each accumulator in AGPRs from one stage to
+  // the next (otherwise the compiler carries them in VGPRs and copies all 192
+  // registers in and out of AGPRs every stage).
+  auto compute = [&](int j) {
+    const double2 *sA = abuf(j);
+    mma3_stage_64x32(sA, sA + kDlA, wm + r16, wn + r16, ks, p1, p2, p3);
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj) asm("" : "+a"(p1[ti][tj]), "+a"(p2[ti][tj]), "+a"(p3[ti][tj]));
+  };
+  if (nfull > 0)
+    issue_full(0);
+  else if (nstage > 0)
+    issue_part(0);
+  settle();
+  // The main loop body is one basic block (the next stage is whole): the
+  // accumulators stay in their AGPRs across iterations, no copies at joins.
+  int st = 0;
+  for (; st + 1 < nfull; ++st) {
+    issue_full(st + 1);
+    compute(st);
+    settle();
+  }
+  if (st < nstage) {  // the last whole stage, then the partial one if any
+    if (st + 1 < nstage) issue_part(st + 1);
+    compute(st);
+    settle();
+    if (st + 1 < nstage) {
+      compute(st + 1);
+      settle();
+    }
+  }
+
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = tm * kBM3 + wm + ti * 16 + ks + 4 * r;
+        const int nn = tn * kBN3 + wn + tj * 16 + r16;
+        if (m < M0 && nn < M1) {
+          const double a1 = p1[ti][tj][r], a2 = p2[ti][tj][r];
+          work[((size_t)chunk * M0 + m) * M1 + nn] = make_double2(a1 + a2, p3[ti][tj][r] - a1 + a2);
+        }
+      }
+}
+
 __global__ void k_contract_reduce(const double2 *__restrict__ work, int nsplit, int M0, int M1,
                                   double2 *__restrict__ out, int ldo, int accumulate) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -638,25 +754,33 @@ __global__ void k_contract_reduce_cols(const double2 *__restrict__ work, int nsp
 // Kernel choice for tables of at least 128 x 128: the three-multiplication
 // kernel (LQ_CONTRACT=4m selects the four-multiplication LDS kernel); smaller
 // tables, or LQ_CONTRACT=wave, use the register-only 64 x 64 kernel.
-enum ContractKind { kWave = 0, kLds4m = 1, kLds3m = 2 };
+enum ContractKind { kWave = 0, kLds4m = 1, kLds3m = 2, kLds3m128 = 3 };
 
+// Kernel choice for tables of at least 128 x 128: the 128 x 128 three-product
+// kernel (module lq_contract_f64_3m_t128).  LQ_CONTRACT=dl selects the
+// 128 x 64 direct-load kernel (also selected by LQ_CONTRACT_DL=0, which takes
+// its register-staged variant, and by LQ_CONTRACT_DLBUF=3), =w1 its one-wave-
+// per-SIMD variant, =4m the four-product kernel; =wave, or a smaller table,
+// the register-only 64 x 64 kernel.
 int contract_kind(int M0, int M1) {
   static const int forced = [] {
     const char *e = getenv("LQ_CONTRACT");
     if (e && strcmp(e, "wave") == 0) return (int)kWave;
     if (e && strcmp(e, "4m") == 0) return (int)kLds4m;
+    if (e && (strcmp(e, "dl") == 0 || strcmp(e, "w1") == 0)) return (int)kLds3m;
+    if (e && strcmp(e, "t128") == 0) return (int)kLds3m128;
+    if (getenv("LQ_CONTRACT_DL") || getenv("LQ_CONTRACT_DLBUF")) return (int)kLds3m;
     return -1;
   }();
   if (M0 < kBT || M1 < kBT) return kWave;
-  return forced >= 0 ? forced : kLds3m;
+  return forced >= 0 ? forced : kLds3m128;
 }
 
 bool use_lds(int M0, int M1) { return contract_kind(M0, M1) != kWave; }
 
-// Three-multiplication kernel with direct global -> LDS loads
-// (k_contract_f64_3m_dl, the default); LQ_CONTRACT_DL=0 selects the
-// register-staged k_contract_f64_3m.  Both compute the same products in the
-// same order.
+// The 128 x 64 three-multiplication kernels: direct global -> LDS loads
+// (k_contract_f64_3m_dl) unless LQ_CONTRACT_DL=0 selects the register-staged
+// k_contract_f64_3m.  Both compute the same products in the same order.
 bool contract_dl() {
   static const bool on = [] {
     const char *e = getenv("LQ_CONTRACT_DL");
@@ -675,6 +799,17 @@ int contract_dlbuf() {
   return nb;
 }
 
+// One-wave-per-SIMD three-product kernel (LQ_CONTRACT=w1): the same tables
+// as k_contract_f64_3m_dl, in 384 registers per SIMD lane, so a recursion
+// wave of up to 128 registers fits beside it on every SIMD.
+bool contract_w1() {
+  static const bool on = [] {
+    const char *e = getenv("LQ_CONTRACT");
+    return e && strcmp(e, "w1") == 0;
+  }();
+  return on;
+}
+
 // XCD-major block order (block_chunk_tile); LQ_CONTRACT_XMAP=0 restores
 // chunk = block % nsplit.
 bool contract_xmap() {
@@ -685,9 +820,40 @@ bool contract_xmap() {
   return on;
 }
 
+// The 128 x 128 kernel is a code object of its own (its register split is
+// set in its LLVM IR at build time), embedded in this library and loaded once
+// per device.
+extern "C" const unsigned char lq_module_contract_t128[];
+
+hipFunction_t t128_function(int device) {
+  static std::mutex mu;
+  static std::map<int, hipFunction_t> fns;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = fns.find(device);
+  if (it != fns.end()) return it->second;
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+  if (hipModuleLoadData(&mod, lq_module_contract_t128) != hipSuccess ||
+      hipModuleGetFunction(&fn, mod, "lq_contract_f64_3m_t128") != hipSuccess)
+    return nullptr;
+  fns[device] = fn;
+  return fn;
+}
+
+// Its direct loads address each lane's row as an unsigned 32-bit byte offset
+// of at most three rows plus 240 bytes from a scalar base.
+bool t128_fits(long long lda, long long ldb) {
+  const long long lim = ((1LL << 32) - 256) / (3 * (long long)sizeof(double2));
+  return lda <= lim && ldb <= lim;
+}
+
+// Split-K chunks.  The 128 x 128 kernel takes the chunks of the 128 x 64
+// kernels (its time does not depend on them: 46.2-46.3 ms at 16, 32 or 64
+// chunks for 512 x 512 x 2^21), so every three-product kernel sums the same
+// chunks in the same order and gives the same tables.
 int choose_nsplit(int M0, int M1, long long K) {
   const int kind = contract_kind(M0, M1);
-  const int bm = kind == kWave ? 64 : kBT, bn = kind == kLds3m ? kBN3 : bm;
+  const int bm = kind == kWave ? 64 : kBT, bn = (kind == kLds3m || kind == kLds3m128) ? kBN3 : bm;
   const int target = kind == kWave ? 2048 : 512;  // blocks in flight
   const long long tiles = (long long)((M0 + bm - 1) / bm) * ((M1 + bn - 1) / bn);
   long long ns = (target + tiles - 1) / tiles;
@@ -725,7 +891,31 @@ int launch_contract_f64(hipStream_t st, const double2 *A, long long lda, int M0,
     return LQ_ERR_ARG;
   }
   const int xmap = (nsplit % 8 == 0 && contract_xmap()) ? 1 : 0;
-  if (kind == kLds3m && contract_dl() && contract_dlbuf() == 2)
+  if (kind == kLds3m128 && t128_fits(lda, ldb)) {
+    int dev = 0;
+    LQ_HIP_CHECK(hipGetDevice(&dev));
+    hipFunction_t fn = t128_function(dev);
+    if (!fn) {
+      set_error("launch_contract_f64: cannot load the lq_contract_f64_3m_t128 code object on device %d", dev);
+      return LQ_ERR_HIP;
+    }
+    int m0 = M0, m1 = M1, ns = nsplit, tn = tiles_n, xm = xmap;
+    long long la = lda, lb = ldb, kk = K, kc = kchunk;
+    const double2 *a = A, *b = B;
+    double2 *w = work;
+    void *args[] = {&a, &la, &m0, &b, &lb, &m1, &kk, &kc, &ns, &tn, &xm, &w};
+    LQ_HIP_CHECK(hipModuleLaunchKernel(fn, (unsigned)blocks, 1, 1, 512, 1, 1, 0, st, args, nullptr));
+    return LQ_OK;
+  }
+  if (kind == kLds3m128) {  // rows too long for 32-bit lane offsets: the 128 x 64 kernel, same tables
+    const int tiles_n64 = (M1 + kBN3 - 1) / kBN3;
+    const long long blocks64 = (long long)tiles_m * tiles_n64 * nsplit;
+    hipLaunchKernelGGL(k_contract_f64_3m_dl<2>, dim3((unsigned)blocks64), dim3(512), 0, st, A, lda, M0, B, ldb, M1,
+                       K, kchunk, nsplit, tiles_n64, xmap, work);
+  } else if (kind == kLds3m && contract_w1())
+    hipLaunchKernelGGL(k_contract_f64_3m_w1, dim3((unsigned)blocks), dim3(256), 0, st, A, lda, M0, B, ldb, M1, K,
+                       kchunk, nsplit, tiles_n, xmap, work);
+  else if (kind == kLds3m && contract_dl() && contract_dlbuf() == 2)
     hipLaunchKernelGGL(k_contract_f64_3m_dl<2>, dim3((unsigned)blocks), dim3(512), 0, st, A, lda, M0, B, ldb, M1, K,
                        kchunk, nsplit, tiles_n, xmap, work);
   else if (kind == kLds3m && contract_dl())

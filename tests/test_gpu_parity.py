@@ -107,9 +107,11 @@ np.save(sys.argv[5], sub)
 
 @pytest.mark.parametrize("ML,MR,K", [(320, 200, 20483), (384, 320, 65536)])
 def test_contraction_kernels_bitwise_equal(tmp_path, ML, MR, K):
-    """The three-product kernels -- direct-to-LDS loads with two buffers (the
-    default) or three (LQ_CONTRACT_DLBUF=3), and register-staged
-    (LQ_CONTRACT_DL=0) -- form the same products in the same order: equal
+    """The three-product kernels -- the 128 x 128 tile kernel loaded as a code
+    object (the default), the 128 x 64 kernels with direct-to-LDS loads into
+    two buffers (LQ_CONTRACT=dl) or three (LQ_CONTRACT_DLBUF=3), with one wave
+    per SIMD (LQ_CONTRACT=w1) or register-staged (LQ_CONTRACT_DL=0) -- form the
+    same products in the same order: equal
     tables, bit for bit, including a partial last K stage and edge tiles.  The
     four-product kernel (LQ_CONTRACT=4m) rounds differently and is held to the
     same tolerance against the product itself.  The kernel is chosen once per
@@ -118,8 +120,8 @@ def test_contraction_kernels_bitwise_equal(tmp_path, ML, MR, K):
 
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     out = {}
-    for tag, extra in (("dl2", {}), ("dl3", {"LQ_CONTRACT_DLBUF": "3"}), ("reg", {"LQ_CONTRACT_DL": "0"}),
-                       ("4m", {"LQ_CONTRACT": "4m"})):
+    for tag, extra in (("t128", {}), ("dl2", {"LQ_CONTRACT": "dl"}), ("dl3", {"LQ_CONTRACT_DLBUF": "3"}),
+                       ("reg", {"LQ_CONTRACT_DL": "0"}), ("w1", {"LQ_CONTRACT": "w1"}), ("4m", {"LQ_CONTRACT": "4m"})):
         say("contraction kernels", ML, MR, K, tag)
         path = tmp_path / f"{tag}.npy"
         env = {k: v for k, v in os.environ.items() if not k.startswith("LQ_CONTRACT")}
@@ -128,13 +130,13 @@ def test_contraction_kernels_bitwise_equal(tmp_path, ML, MR, K):
                            env=env, capture_output=True, text=True, timeout=300)
         assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
         out[tag] = np.load(path)
-    assert np.array_equal(out["dl2"], out["reg"])
-    assert np.array_equal(out["dl3"], out["reg"])
+    for tag in ("t128", "dl2", "dl3", "w1"):
+        assert np.array_equal(out[tag], out["reg"]), tag
     rng = np.random.default_rng(5)
     L = rng.normal(size=(ML, K)) + 1j * rng.normal(size=(ML, K))
     R = rng.normal(size=(MR, K)) + 1j * rng.normal(size=(MR, K))
     ref = L.conj() @ R.T
-    assert rel_inf(out["dl2"], ref) < 1e-13
+    assert rel_inf(out["t128"], ref) < 1e-13
     assert rel_inf(out["4m"], ref) < 1e-13
 
 

@@ -398,8 +398,10 @@ __global__ __launch_bounds__(512) void k_contract_f64_3m(const double2 *__restri
 // stage st starts, and one counter wait plus one barrier per stage make stage
 // st + 1 visible.  These loads are fetched from beyond L2 about twice as often
 // as the register-staged kernel's (measured; the cache-policy bits did not
-// change that), yet the kernel is faster: it is bound by the MFMA pipe.
-// LDS rows are 16 unpadded slots; element k of row R sits in slot
+// change that), yet the kernel is faster.  At 81-84 % of the FP64 peak it
+// loses most of the rest to that per-stage wait and barrier (measured by
+// removing each in turn), which the 128 x 128 tile of lq_contract_f64_3m_t128
+// halves per flop.  LDS rows are 16 unpadded slots; element k of row R sits in slot
 // k ^ (R & 15), so the 16 rows read by one MFMA operand fetch hit 16
 // different slots and a 16-lane load group still covers one row's 256
 // contiguous bytes.  A partial last stage is loaded through registers with
@@ -575,13 +577,15 @@ __global__ __launch_bounds__(512) void k_contract_f64_3m_dl(const double2 *__res
 
 // One wave per SIMD: the same block tile (128 x 64), LDS layout, direct loads
 // and MFMA sequence per output element as k_contract_f64_3m_dl, with four
-// waves of 64 x 32 (ti = 0..3, tj = 0..1) instead of eight of 32 x 32.  Each
-// wave does twice the MFMA work between two barriers, and its 96 accumulator
-// registers x 2 live in AGPRs, so a block holds about 3/5 of a SIMD lane's
-// registers and leaves room for a wave of another kernel (the recursion
-// step) on every SIMD.  Each output element sees the same products in the
-// same K order as in k_contract_f64_3m_dl: the tables are bitwise equal.
-
+// waves of 64 x 32 (ti = 0..3, tj = 0..1) instead of eight of 32 x 32.  Its
+// 192 accumulator registers live in AGPRs (amdgpu_num_vgpr(192) caps both
+// register files at 192), so a block holds 3/4 of a SIMD lane's registers and
+// leaves room for a recursion wave of up to 128 registers on every SIMD.
+// Each output element sees the same products in the same K order as in
+// k_contract_f64_3m_dl: the tables are bitwise equal.  Measured 72 % of the
+// FP64 peak alone (one wave cannot hide its own LDS latency), and a recursion
+// beside it advances at about a quarter of its speed, so it is a schedule
+// option (LQ_CONTRACT=w1), not the default.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_num_vgpr(192))) void k_contract_f64_3m_w1(const double2 *__restrict__ A, long long lda, int M0,
                                                             const double2 *__restrict__ B, long long ldb, int M1,
                                                             long long K, long long kchunk, int nsplit, int tiles_n,

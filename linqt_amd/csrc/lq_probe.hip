@@ -88,17 +88,21 @@ int lq_hbm_probe(lq_ctx ctx, size_t bytes, int kind, int reps, double *ms, doubl
   hipLaunchKernelGGL(k_probe_write, dim3(grid), dim3(kProbeThreads), 0, st, buf, words, 7u);  // touch every page
   LQ_HIP_CHECK(hipGetLastError());
   LQ_TRY(launch(1));  // warm-up
-  hipEvent_t a, b;
-  LQ_HIP_CHECK(hipEventCreate(&a));
-  LQ_HIP_CHECK(hipEventCreate(&b));
-  LQ_HIP_CHECK(hipEventRecord(a, st));
+  struct Events {  // destroyed on every return path
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Events() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } ev;
+  LQ_HIP_CHECK(hipEventCreate(&ev.a));
+  LQ_HIP_CHECK(hipEventCreate(&ev.b));
+  LQ_HIP_CHECK(hipEventRecord(ev.a, st));
   for (int r = 0; r < reps; ++r) LQ_TRY(launch(2 + r));
-  LQ_HIP_CHECK(hipEventRecord(b, st));
-  LQ_HIP_CHECK(hipEventSynchronize(b));
+  LQ_HIP_CHECK(hipEventRecord(ev.b, st));
+  LQ_HIP_CHECK(hipEventSynchronize(ev.b));
   float t = 0.f;
-  LQ_HIP_CHECK(hipEventElapsedTime(&t, a, b));
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
+  LQ_HIP_CHECK(hipEventElapsedTime(&t, ev.a, ev.b));
   *ms = (double)t / reps;
   *moved = (double)(words * sizeof(u32x4));  // read, read + write of halves, or written
   return LQ_OK;

@@ -845,9 +845,16 @@ hipFunction_t t128_function(int device) {
 }
 
 // Its direct loads address each lane's row as an unsigned 32-bit byte offset
-// of at most three rows plus 240 bytes from a scalar base.
+// of at most three rows plus 240 bytes from a scalar base, so rows longer than
+// lim elements take the 128 x 64 kernel (same tables).  LQ_CONTRACT_T128_MAXLD
+// lowers the limit (to test that path at small sizes).
 bool t128_fits(long long lda, long long ldb) {
-  const long long lim = ((1LL << 32) - 256) / (3 * (long long)sizeof(double2));
+  static const long long lim = [] {
+    long long l = ((1LL << 32) - 256) / (3 * (long long)sizeof(double2));
+    const char *e = getenv("LQ_CONTRACT_T128_MAXLD");
+    if (e && atoll(e) > 0 && atoll(e) < l) l = atoll(e);
+    return l;
+  }();
   return lda <= lim && ldb <= lim;
 }
 

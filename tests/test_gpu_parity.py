@@ -110,8 +110,10 @@ def test_contraction_kernels_bitwise_equal(tmp_path, ML, MR, K):
     """The three-product kernels -- the 128 x 128 tile kernel loaded as a code
     object (the default), the 128 x 64 kernels with direct-to-LDS loads into
     two buffers (LQ_CONTRACT=dl) or three (LQ_CONTRACT_DLBUF=3), with one wave
-    per SIMD (LQ_CONTRACT=w1) or register-staged (LQ_CONTRACT_DL=0) -- form the
-    same products in the same order: equal
+    per SIMD (LQ_CONTRACT=w1) or register-staged (LQ_CONTRACT_DL=0), and the
+    128 x 64 kernel the default falls back to for rows too long for its 32-bit
+    lane offsets (forced with a lowered limit) -- form the same products in the
+    same order: equal
     tables, bit for bit, including a partial last K stage and edge tiles.  The
     four-product kernel (LQ_CONTRACT=4m) rounds differently and is held to the
     same tolerance against the product itself.  The kernel is chosen once per
@@ -121,7 +123,8 @@ def test_contraction_kernels_bitwise_equal(tmp_path, ML, MR, K):
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     out = {}
     for tag, extra in (("t128", {}), ("dl2", {"LQ_CONTRACT": "dl"}), ("dl3", {"LQ_CONTRACT_DLBUF": "3"}),
-                       ("reg", {"LQ_CONTRACT_DL": "0"}), ("w1", {"LQ_CONTRACT": "w1"}), ("4m", {"LQ_CONTRACT": "4m"})):
+                       ("reg", {"LQ_CONTRACT_DL": "0"}), ("w1", {"LQ_CONTRACT": "w1"}), ("4m", {"LQ_CONTRACT": "4m"}),
+                       ("longrows", {"LQ_CONTRACT_T128_MAXLD": str(K - 1)})):
         say("contraction kernels", ML, MR, K, tag)
         path = tmp_path / f"{tag}.npy"
         env = {k: v for k, v in os.environ.items() if not k.startswith("LQ_CONTRACT")}
@@ -130,7 +133,7 @@ def test_contraction_kernels_bitwise_equal(tmp_path, ML, MR, K):
                            env=env, capture_output=True, text=True, timeout=300)
         assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
         out[tag] = np.load(path)
-    for tag in ("t128", "dl2", "dl3", "w1"):
+    for tag in ("t128", "dl2", "dl3", "w1", "longrows"):
         assert np.array_equal(out[tag], out["reg"]), tag
     rng = np.random.default_rng(5)
     L = rng.normal(size=(ML, K)) + 1j * rng.normal(size=(ML, K))

@@ -9,7 +9,10 @@
 //
 // * Admission: jobs queue in submission order and are admitted first-fit at
 //   chunk boundaries while their device footprint (states, two recursion
-//   vectors, dot partials, moments) fits the batch's HBM budget.
+//   vectors, dot partials, moments) fits the batch's HBM budget.  The budget
+//   bounds all device memory the batch holds: the blocks of running jobs (a
+//   reused block may be larger than its job needs) and the blocks kept for
+//   reuse.
 // * Determinism: a job's block count, row schedule, dot partial layout and
 //   reduction order depend on the job alone, so its moments are bitwise the
 //   same alone, in any mix, and through lq_spectral_moments (R <= 64).
@@ -45,7 +48,8 @@ struct Job {
   std::vector<double> phi_host;
   const double *phi_dev = nullptr;
   void *mem = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0;  // device footprint of the job
+  size_t held = 0;   // size of the block it runs in (a reused block may be larger)
   BatchJobDev d{};
   std::vector<std::complex<double>> result;
 };
@@ -87,7 +91,8 @@ struct lq_batch_s {
   // Device blocks of finished jobs kept for reuse (best fit), so admitting and
   // harvesting jobs calls hipMalloc only when no kept block fits and hipFree
   // (which waits for the whole device, other streams included) only when more
-  // than kKeepBytes would be kept.
+  // than kKeepBytes would be kept or a new block would exceed the budget.
+  // used (blocks of running jobs) + spare_bytes <= budget at all times.
   std::multimap<size_t, void *> spare;
   size_t spare_bytes = 0;
 };
@@ -96,15 +101,38 @@ namespace {
 
 constexpr size_t kKeepBytes = (size_t)4 << 30;
 
-int block_get(lq_batch b, size_t bytes, void **p) {
+// A kept block that fits (best fit, at most twice the need + 64 MiB): it moves
+// from the kept blocks to the running ones, so the batch's memory stays the same.
+std::multimap<size_t, void *>::iterator spare_fit(lq_batch b, size_t bytes) {
   auto it = b->spare.lower_bound(bytes);
-  if (it != b->spare.end() && it->first <= 2 * bytes + ((size_t)64 << 20)) {  // best fit, not wasteful
+  if (it != b->spare.end() && it->first <= 2 * bytes + ((size_t)64 << 20)) return it;
+  return b->spare.end();
+}
+
+// Whether a job of `bytes` can be admitted now without the batch holding more
+// than its budget (kept blocks may be freed to make room).
+bool block_admissible(lq_batch b, size_t bytes) {
+  return spare_fit(b, bytes) != b->spare.end() || b->used + bytes <= b->budget;
+}
+
+int block_get(lq_batch b, size_t bytes, void **p, size_t *held) {
+  auto it = spare_fit(b, bytes);
+  if (it != b->spare.end()) {
     *p = it->second;
+    *held = it->first;
     b->spare_bytes -= it->first;
     b->spare.erase(it);
     return LQ_OK;
   }
+  while (b->used + b->spare_bytes + bytes > b->budget && !b->spare.empty()) {  // make room: largest first
+    auto big = std::prev(b->spare.end());
+    b->spare_bytes -= big->first;
+    void *q = big->second;
+    b->spare.erase(big);
+    LQ_HIP_CHECK(hipFree(q));
+  }
   LQ_HIP_CHECK(hipMalloc(p, bytes));
+  *held = bytes;
   return LQ_OK;
 }
 
@@ -123,7 +151,7 @@ int block_put(lq_batch b, void *p, size_t bytes) {
 
 int admit(lq_batch b, Job &j) {
   lq_ctx ctx = b->ctx;
-  LQ_TRY(block_get(b, j.bytes, &j.mem));
+  LQ_TRY(block_get(b, j.bytes, &j.mem, &j.held));
   char *p = (char *)j.mem;
   const size_t vec = (size_t)j.n * j.R * sizeof(double2);
   BatchJobDev &d = j.d;
@@ -154,7 +182,7 @@ int admit(lq_batch b, Job &j) {
   LQ_TRY(launch_dot(ctx->stream, j.n, j.R, d.phi, d.buf0, d.partial, &grid));
   LQ_TRY(launch_reduce_partials(ctx->stream, d.partial, 1, grid, j.kp, d.mom));
   j.state = 1;
-  b->used += j.bytes;
+  b->used += j.held;
   return LQ_OK;
 }
 
@@ -169,9 +197,9 @@ int harvest(lq_batch b, Job &j) {
   }
   void *m = j.mem;
   j.mem = nullptr;
-  b->used -= j.bytes;
+  b->used -= j.held;
   j.state = 2;
-  LQ_TRY(block_put(b, m, j.bytes));
+  LQ_TRY(block_put(b, m, j.held));
   return LQ_OK;
 }
 
@@ -180,7 +208,7 @@ int run_chunk(lq_batch b) {
   // admission, first fit in submission order
   for (size_t q = 0; q < b->queue.size();) {
     Job &j = b->jobs[b->queue[q]];
-    if (b->used + j.bytes <= b->budget) {
+    if (block_admissible(b, j.bytes)) {
       LQ_TRY(admit(b, j));
       b->queue.erase(b->queue.begin() + q);
     } else {

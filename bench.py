@@ -52,7 +52,7 @@ FP64_MFMA_PEAK_TFS = 78.6
 INT8_MFMA_PEAK_TOPS = 5000.0
 FP32_MFMA_PEAK_TFS = 157.3
 BF16_MFMA_PEAK_TFS = 2500.0
-PRECISIONS = {"fp64": 0, "fp32x2": 2, "bf16x3": 3, "int8x4": 4, "int8x5": 5, "int8x6": 6, "int8x7": 7}
+PRECISIONS = {"fp64": 0, "fp32x2": 2, "bf16x3": 3, "int8x4": 4, "int8x5": 5, "int8x6": 6, "int8x7": 7, "int8x8": 8}
 
 
 def kernel_source_sha16():

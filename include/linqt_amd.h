@@ -72,7 +72,8 @@ enum lq_precision {
   LQ_PREC_INT8X4 = 4, /* S = 4 */
   LQ_PREC_INT8X5 = 5, /* S = 5 */
   LQ_PREC_INT8X6 = 6, /* S = 6 */
-  LQ_PREC_INT8X7 = 7  /* S = 7 */
+  LQ_PREC_INT8X7 = 7, /* S = 7 */
+  LQ_PREC_INT8X8 = 8  /* S = 8 */
 };
 
 typedef struct lq_ctx_s *lq_ctx;

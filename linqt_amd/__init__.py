@@ -29,7 +29,7 @@ import numpy as np
 from . import _lib
 from ._lib import (LQ_ALGO_DIRECT, LQ_ALGO_DOUBLING, LQ_FILL_GENERAL, LQ_FILL_HERMITIAN_UPPER,
                    LQ_KUBO_BASTIN, LQ_KUBO_GREENWOOD, LQ_PREC_FP64, LQ_PREC_FP32X2, LQ_PREC_BF16X3, LQ_PREC_INT8X4, LQ_PREC_INT8X5,
-                   LQ_PREC_INT8X6, LQ_PREC_INT8X7, LQ_FMT_TEXT, LQ_FMT_TEXT_REF, LQ_FMT_BINARY, LinqtError)
+                   LQ_PREC_INT8X6, LQ_PREC_INT8X7, LQ_PREC_INT8X8, LQ_FMT_TEXT, LQ_FMT_TEXT_REF, LQ_FMT_BINARY, LinqtError)
 
 __all__ = [
     "Context", "SparseMatrix", "DeviceStates", "Batch", "read_csr_file", "write_csr_file", "read_mom_file",
@@ -38,6 +38,7 @@ __all__ = [
     "jackson_1d", "jackson_2d", "dos_fftgrid", "greenwood_fftgrid", "bastin_fftgrid", "LinqtError",
     "LQ_FILL_HERMITIAN_UPPER", "LQ_FILL_GENERAL", "LQ_KUBO_GREENWOOD", "LQ_KUBO_BASTIN", "LQ_ALGO_DIRECT",
     "LQ_ALGO_DOUBLING", "LQ_PREC_FP64", "LQ_PREC_FP32X2", "LQ_PREC_BF16X3", "LQ_PREC_INT8X4", "LQ_PREC_INT8X5", "LQ_PREC_INT8X6", "LQ_PREC_INT8X7",
+    "LQ_PREC_INT8X8",
 ]
 
 

@@ -34,6 +34,7 @@ LQ_PREC_INT8X4 = 4
 LQ_PREC_INT8X5 = 5
 LQ_PREC_INT8X6 = 6
 LQ_PREC_INT8X7 = 7
+LQ_PREC_INT8X8 = 8
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

@@ -204,9 +204,9 @@ int launch_slice(hipStream_t st, const double2 *X, long long ldx, int rows, long
                  int *E);
 int launch_contract_i8(hipStream_t st, const signed char *QA, const int *EA, int M0, const signed char *QB,
                        const int *EB, int M1, long long K, int S, double2 *work, int nsplit);
-inline bool precision_ok(int p) { return p == 0 || p == 2 || p == 3 || (p >= 4 && p <= 7); }
-// Split precision with int8 digit planes (LQ_PREC_INT8X4..X7).
-inline bool precision_int8(int p) { return p >= 4 && p <= 7; }
+inline bool precision_ok(int p) { return p == 0 || p == 2 || p == 3 || (p >= 4 && p <= 8); }
+// Split precision with int8 digit planes (LQ_PREC_INT8X4..X8).
+inline bool precision_int8(int p) { return p >= 4 && p <= 8; }
 // Floating-point split forms (lq_split.hip): FP32 hi/lo or 3 x BF16 pieces,
 // FP32 MFMA accumulation folded into FP64 every 256 K-elements.
 int split_nsplit(int M0, int M1, long long K);

@@ -308,9 +308,10 @@ int launch_slice(hipStream_t st, const double2 *X, long long ldx, int rows, long
     LQ_SL(5)
     LQ_SL(6)
     LQ_SL(7)
+    LQ_SL(8)
 #undef LQ_SL
     default:
-      set_error("launch_slice: %d slices unsupported (4..7)", S);
+      set_error("launch_slice: %d slices unsupported (4..8)", S);
       return LQ_ERR_ARG;
   }
   LQ_HIP_CHECK(hipGetLastError());
@@ -346,9 +347,10 @@ int launch_contract_i8(hipStream_t st, const signed char *QA, const int *EA, int
     LQ_CI(5)
     LQ_CI(6)
     LQ_CI(7)
+    LQ_CI(8)
 #undef LQ_CI
     default:
-      set_error("launch_contract_i8: %d slices unsupported (4..7)", S);
+      set_error("launch_contract_i8: %d slices unsupported (4..8)", S);
       return LQ_ERR_ARG;
   }
   LQ_HIP_CHECK(hipGetLastError());

@@ -18,7 +18,7 @@ import ozaki_model as oz
 from conftest import rel_inf
 from linqt_amd import lattices
 
-SLICES = [4, 5, 6, 7]
+SLICES = [4, 5, 6, 7, 8]
 
 
 def _operands(M0, M1, K, seed, spread=True):
@@ -39,7 +39,9 @@ def test_model_within_bound(S):
     A, B = _operands(6, 5, 3 * oz.GROUP + 100, seed=S)
     exact = A.conj() @ B.T
     got = oz.contract(A, B, S)
-    bound = oz.error_bound(A, B, S)
+    # at 8 digits the bound falls below the FP64 rounding of the exact product
+    # and of the model's fold: allow that rounding (a few ulps of the magnitude)
+    bound = oz.error_bound(A, B, S) + 4e-15 * oz.magnitude(A, B)
     err_r, err_i = np.abs((got - exact).real), np.abs((got - exact).imag)
     assert np.all(err_r <= bound) and np.all(err_i <= bound)
 
@@ -132,7 +134,7 @@ def test_mom_table_fp64_precision_matches_plain(ctx):
     A, B = _operands(40, 30, 5000, seed=3, spread=False)
     got = _table_gpu(ctx, A, B, 0)
     assert rel_inf(got, A.conj() @ B.T) < 1e-12
-    for bad in (1, 8, -1):
+    for bad in (1, 9, -1):
         with pytest.raises(lq.LinqtError):
             _table_gpu(ctx, A, B, bad)
 
@@ -168,7 +170,7 @@ def test_kubo_split_precision_vs_fp64(ctx):
     ref = lq.kubo_fft(G[0], G[1], [G[1], G[2]], phi, M, nump, lq.LQ_KUBO_BASTIN)
     sig_ref = [lq.kubo_post(ctx, 1, ref[o], H.n, 9.2, 0.0)[1] for o in range(2)]
     errs = {}
-    for S in (4, 5, 6, 7):
+    for S in (4, 5, 6, 7, 8):
         say("kubo split", S)
         got = lq.kubo_fft(G[0], G[1], [G[1], G[2]], phi, M, nump, lq.LQ_KUBO_BASTIN, precision=S)
         for o in range(2):
@@ -186,3 +188,4 @@ def test_kubo_split_precision_vs_fp64(ctx):
         assert errs[(7, o)] < 1e-8  # the accuracy target of the split path
         assert errs[(6, o)] < 1e-7 and errs[(5, o)] < 1e-4 and errs[(4, o)] < 1e-2
         assert errs[(7, o)] < errs[(6, o)] < errs[(5, o)] < errs[(4, o)]
+        assert errs[(8, o)] < min(errs[(7, o)], 1e-9)

@@ -1796,7 +1796,9 @@ bool tune_lookup(const std::string &key, int *u, int *gc, int *pf) {
     if (rest.size() > key.size() && rest.compare(0, key.size(), key) == 0 && rest[key.size()] == ' ') {
       std::istringstream is(rest.substr(key.size()));
       int a, b, c;
-      if (is >> a >> b >> c) {
+      // only schedules the tuner can choose (a damaged line is skipped)
+      if ((is >> a >> b >> c) && (a == 4 || a == 8 || a == 12 || a == 16) && (b == 0 || b >= 256) &&
+          (c == 0 || c == 1)) {
         *u = a;
         *gc = b;
         *pf = c;

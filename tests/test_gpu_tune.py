@@ -46,6 +46,13 @@ def test_tuned_schedules_keep_results_and_persist(ctx, tmp_path, monkeypatch):
         {k: t0[k] for k in ("unroll", "gridcap", "prefetch")}
     lines = [ln for ln in cache.read_text().splitlines() if ln.startswith("lqtune1 ")]
     assert len(lines) == 3
+    # a damaged entry for the same key (appended last) is skipped, the valid one still read
+    key = " ".join(lines[1].split()[1:-4])
+    with open(cache, "a") as f:
+        f.write(f"lqtune1 {key} 5 -3 7 1.0\n")
+    r2 = H2.tune(8, mode=0)
+    assert r2["from_cache"] and {k: r2[k] for k in ("unroll", "gridcap", "prefetch")} == \
+        {k: t0[k] for k in ("unroll", "gridcap", "prefetch")}
     with pytest.raises(lq.LinqtError):
         H.tune(0)
 

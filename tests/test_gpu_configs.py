@@ -6,6 +6,8 @@
   columns of <L_m|R_n> from its own recursion), at M = 1024 the same; the
   split-precision sigma error against the FP64 path at M = 1024 is printed
   and bounded.
+* The same M = 512 tables (R = 4) equal, bit for bit, between the default
+  128 x 128 contraction kernel and the 128 x 64 one.
 * Batched DOS jobs, one per size class: cubic Anderson 16^3, 32^3, 64^3,
   96^3 and Kane-Mele-like honeycombs of N = 16384 and N = 1048576 in one
   batch, each bitwise equal to its single-job solve; cubic 16^3 also
@@ -105,6 +107,45 @@ def test_kubo_bastin_table_full_M_vs_oracle_rows_and_columns(ctx, oracle, honeyc
         ec = rel_inf(got[o][:, :nb], ref_c)
         say(f"  operator {o}: rows {er:.2e}, columns {ec:.2e}")
         assert er < 1e-10 and ec < 1e-10, (o, er, ec)
+
+
+_DL_SCRIPT = r"""
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import linqt_amd as lq
+from linqt_amd import lattices
+ctx = lq.Context(0)
+H, Vx, Vy = lattices.haldane_honeycomb(512, seed=20261203)
+G = [lq.SparseMatrix.from_csr(ctx, H).Rescale(1 / 4.6, 0.0), lq.SparseMatrix.from_csr(ctx, Vx),
+     lq.SparseMatrix.from_csr(ctx, Vy)]
+phi = lattices.random_phase_states(H.n, 4, seed=20261203)
+mu = lq.correlation_expansion_moments(G[0], G[1], [G[1], G[2]], phi, 512, 512)
+np.savez(sys.argv[2], xx=mu[0], xy=mu[1])
+"""
+
+
+def test_c2_tables_equal_across_contraction_kernels(ctx, honeycomb512, tmp_path):
+    """At the benchmark's own size (N = 524288, M = 512, R = 4: tables of
+    512 x 512 over K = 2^21) the default 128 x 128 kernel and the 128 x 64
+    kernel (LQ_CONTRACT=dl, in a child process: the kernel is chosen once per
+    process) sum the same split-K chunks in the same order: equal tables."""
+    import os
+    import subprocess
+
+    n, G, O, phi = honeycomb512
+    say(f"corr N={n} M=512 R=4: default contraction kernel")
+    got = lq.correlation_expansion_moments(G[0], G[1], [G[1], G[2]], phi, 512, 512)
+    say("same with LQ_CONTRACT=dl (child process)")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = {k: v for k, v in os.environ.items() if not k.startswith("LQ_CONTRACT")}
+    env["LQ_CONTRACT"] = "dl"
+    path = tmp_path / "dl.npz"
+    r = subprocess.run([sys.executable, "-c", _DL_SCRIPT, root, str(path)], env=env, capture_output=True,
+                       text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    ref = np.load(path)
+    assert np.array_equal(got[0], ref["xx"]) and np.array_equal(got[1], ref["xy"])
 
 
 def test_split_precision_sigma_error_at_M1024(ctx, honeycomb512):

@@ -402,6 +402,8 @@ def main():
     ap.add_argument("--M", type=int, default=None)
     ap.add_argument("--R", type=int, default=None)
     ap.add_argument("--nump", type=int, default=2048)
+    ap.add_argument("--algo", choices=("direct", "doubling"), default="direct",
+                    help="1D moment algorithm of the dos workload (doubling: OP = identity, half the recursion steps)")
     ap.add_argument("--precision", choices=tuple(PRECISIONS), default="fp64",
                     help="contraction precision of the 2D table (int8xS: S int8 digits on INT8 MFMA)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -543,14 +545,17 @@ def main():
         dphi = lq.DeviceStates(ctx, phi)
         log(f"dos N={n} M={M} R={R}")
 
+        algo = lq.LQ_ALGO_DOUBLING if args.algo == "doubling" else lq.LQ_ALGO_DIRECT
+
         def step():
-            mu = lq.spectral_moments(Hg, None, dphi, M)
+            mu = lq.spectral_moments(Hg, None, dphi, M, algo=algo)
             muj = lq.jackson_1d(ctx, mu, 9.2, 1e-3)
             return lq.dos_fftgrid(ctx, muj, n, 9.2, 0.0, nump)
 
         moments_per_step = M * R
-        config = {"workload": f"kpm_dos square {L}x{L} N={n} W=1 M={M} R={R} jackson dos_energies={nump}",
-                  "N": n, "M": M, "R": R, "nump": nump, "precision": "fp64"}
+        config = {"workload": f"kpm_dos square {L}x{L} N={n} W=1 M={M} R={R} jackson dos_energies={nump}"
+                              + (" doubling" if args.algo == "doubling" else ""),
+                  "N": n, "M": M, "R": R, "nump": nump, "precision": "fp64", "algo": args.algo}
 
     for w in range(args.warmup):
         step()

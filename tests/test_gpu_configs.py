@@ -125,7 +125,7 @@ np.savez(sys.argv[2], xx=mu[0], xy=mu[1])
 """
 
 
-def test_c2_tables_equal_across_contraction_kernels(ctx, honeycomb512, tmp_path):
+def test_bastin_512_tables_equal_across_contraction_kernels(ctx, honeycomb512, tmp_path):
     """At the benchmark's own size (N = 524288, M = 512, R = 4: tables of
     512 x 512 over K = 2^21) the default 128 x 128 kernel and the 128 x 64
     kernel (LQ_CONTRACT=dl, in a child process: the kernel is chosen once per

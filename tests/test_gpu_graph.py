@@ -42,8 +42,8 @@ S2 = lattices.square_anderson(256, W=1.0, seed=15)
 H2 = lq.SparseMatrix.from_csr(ctx, S2).Rescale(1 / 4.6, 0.0)
 phi64 = lq.DeviceStates(ctx, lattices.random_phase_states(S2.n, 64, seed=16))
 for rep in range(3):
-    out[f"c1_{rep}"] = lq.spectral_moments(H, None, phi8, 1024)
-    out[f"c1again_{rep}"] = lq.spectral_moments(H, None, phi8, 1024)  # the same solve again: whole loop captured
+    out[f"dos1024_{rep}"] = lq.spectral_moments(H, None, phi8, 1024)
+    out[f"dos1024again_{rep}"] = lq.spectral_moments(H, None, phi8, 1024)  # the same solve again: whole loop captured
     out[f"ring_{rep}"] = lq.spectral_moments(H2, None, phi64, 300)   # partials above 256 MiB: 64-step ring chunks
     out[f"m200_{rep}"] = lq.spectral_moments(H, None, phi8, 200)
     y = H.Multiply(1.0, x)          # eager work between solves
@@ -80,8 +80,8 @@ def test_graph_replays_equal_eager_launches(runs):
         assert np.array_equal(g[key], e[key]), key
         assert np.array_equal(runs["ring_only"][key], e[key]), key
     # every repetition of a solve gives the same moments (cached graph replayed)
-    for name in ("c1", "c1again", "ring", "m200", "op", "r70", "m100"):
+    for name in ("dos1024", "dos1024again", "ring", "m200", "op", "r70", "m100"):
         for rep in (1, 2):
             assert np.array_equal(g[f"{name}_{rep}"], g[f"{name}_0"]), (name, rep)
-    assert np.array_equal(g["c1again_0"], g["c1_0"])
-    assert abs(g["c1_0"][0] - 1.0) < 1e-12
+    assert np.array_equal(g["dos1024again_0"], g["dos1024_0"])
+    assert abs(g["dos1024_0"][0] - 1.0) < 1e-12
